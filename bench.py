@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Throughput of the SPH step on one MI355X: the 3D dam break at 1,024,000
+fluid particles (80 x 80 x 160 at dx = 0.0075, 3-layer fixed boundary),
+mixed precision (FP32 pair physics, FP64 positions/velocities) and FP64.
+
+One "step" is one full leap-frog step of the hot path: reference-grid
+bounding box, cell keys and sort, neighbour search, EOS, the fused force pass
+and the integrator update, with every particle resident in HBM.
+
+Prints ONE JSON line. ``value`` is the mixed-precision particle-steps/s
+(fluid particles x steps / s); the FP64 figure, their ratio, the density
+difference between them, the force kernel's roofline and a CPU baseline
+(the oracle restatement on the host cores, on a bounded sample) ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @1M 3D dam break, mixed vs FP64; p50/p99 batched-run latency"
+PEAK_FP64 = 78.6     # TFLOP/s, MI355X vector FP64
+PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32
+FLOP_PER_PAIR_3D = 81  # algorithmic FLOP per interacting pair (3D dam-break physics set)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(case, precision, warmup, steps):
+    from mysph_amd import config as C
+    from mysph_amd.sphgpu import Simulation
+    cfg = case["config"].copy()
+    cfg.precision = precision
+    sim = Simulation(case, cfg)
+    sim.advance(warmup)
+    try:
+        import torch
+        torch.cuda.synchronize()
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    t1 = time.perf_counter()
+    adv_ms, force_ms, _ = sim.timing()
+    stats = sim.neighbour_stats()
+    rho = sim.get("rho")[: case["n_real"]]
+    sim.close()
+    return dict(wall_s=t1 - t0, event_ms=adv_ms, force_ms=force_ms, pairs=stats["pairs"],
+                max_nbr=stats["max"], rho=rho, label=C.FP64 if precision == 0 else C.MIXED)
+
+
+def cpu_baseline(case, seconds_target=20.0):
+    """The oracle (C restatement of the reference path, OpenMP) on the host,
+    over a bounded number of full steps of the same workload."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    sim = oracle.OracleSim(case)
+    t0 = time.perf_counter()
+    sim.run(1)                      # warm-up step (step 1)
+    t1 = time.perf_counter()
+    per = max(t1 - t0, 1e-3)
+    nsteps = int(max(2, min(20, seconds_target / per)))
+    t2 = time.perf_counter()
+    sim.run(nsteps)
+    t3 = time.perf_counter()
+    return dict(value=case["n_real"] * nsteps / (t3 - t2), unit="particle-steps/s",
+                cores=threads, kind="port",
+                sample=f"the full workload ({case['n_real']} fluid + {case['n_dummy']} boundary), "
+                       f"1 warm-up + {nsteps} timed leap-frog steps, oracle/sph_oracle.c with "
+                       f"{threads} OpenMP threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--dx", type=float, default=0.0075)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design note, not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mysph_amd import cases
+    from mysph_amd import config as C
+    case = cases.dam_break_3d(dx=args.dx)
+    nf = case["n_real"]
+    log(f"case: {nf} fluid + {case['n_dummy']} boundary particles")
+    res = {}
+    for name, prec in (("fp64", C.FP64), ("mixed", C.MIXED)):
+        if args.only not in ("both", name):
+            continue
+        r = run_gpu(case, prec, args.warmup, args.steps)
+        r["pss"] = nf * args.steps / r["wall_s"]
+        res[name] = r
+        log(f"{name}: {r['pss']:.4g} particle-steps/s, {1e3 * r['wall_s'] / args.steps:.3f} ms/step, "
+            f"force kernel {r['force_ms']:.3f} ms, pairs {r['pairs']}")
+    head = res.get("mixed") or res["fp64"]
+    peak = PEAK_FP32 if head is res.get("mixed") else PEAK_FP64
+    flops = FLOP_PER_PAIR_3D * head["pairs"]
+    achieved = flops / (head["force_ms"] * 1e-3) / 1e12 if head["force_ms"] > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": head["pss"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * head["wall_s"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair physics + f64 positions/velocities" if "mixed" in res else "f64",
+        "data": "synthetic (seeded jittered lattice, no dataset)",
+        "config": {
+            "workload": f"3D dam break: 0.6x0.6x1.2 m column, 1.6x0.6 m tank with 1.3 m walls, "
+                        f"dx={args.dx}, {nf} fluid + {case['n_dummy']} fixed boundary particles, "
+                        "h=1.2dx cubic spline, Tait c0=10*sqrt(2gH), Monaghan alpha=0.1 beta=0, "
+                        "continuity density, leap-frog, fixed dt",
+            "fluid_particles": nf,
+            "boundary_particles": case["n_dummy"],
+            "dt": float(case["config"].dt),
+        },
+        "roofline": {
+            "kernel": "k_force_fluid (fused continuity + pressure + Monaghan viscosity pass)",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "flop_per_pair": FLOP_PER_PAIR_3D,
+            "pairs_per_launch": head["pairs"],
+            "launch_ms": head["force_ms"],
+        },
+    }
+    if "fp64" in res:
+        out["fp64"] = {"value": res["fp64"]["pss"],
+                       "ms_per_step": 1e3 * res["fp64"]["wall_s"] / args.steps,
+                       "force_kernel_ms": res["fp64"]["force_ms"]}
+    if "fp64" in res and "mixed" in res:
+        out["mixed_over_fp64"] = res["mixed"]["pss"] / res["fp64"]["pss"]
+        d = np.max(np.abs(res["mixed"]["rho"] - res["fp64"]["rho"]) / res["fp64"]["rho"])
+        out["mixed_rho_rel_err_vs_fp64"] = float(d)
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(case)
+        except Exception as e:  # the GPU figures stand on their own
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,206 @@
+/*
+ * sphgpu.h - C ABI of the MI355X SPH hot path (libsphgpu.so).
+ *
+ * The library replaces the per-step subtree of the reference solver:
+ * the external subroutine single_step(ntotal, ndummy, nbuffer, Particles,
+ * Delta, aver_v, Shear, dSdt) (src/TimeIntegration/singleStep.f90:1-211,
+ * interface src/TimeIntegration/timeIntegration.f90:7-19) and, inside it, the
+ * src/NNPS and src/Physic calls listed at singleStep.f90:58-144, plus the
+ * per-particle update loops of leap_frog (timeIntegration.f90:137-252).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Arrays are particle-major: a vector field
+ *    is x[i*dim + d], a tensor field is s[(i*dim + d)*dim + e]. That is the
+ *    memory order of Fortran x(dim, N) and Stress(dim, dim, N) with d and e
+ *    swapped for tensors (Fortran S(d,e,i) sits at s[(i*dim + e)*dim + d]);
+ *    the Fortran shim passes c_loc of contiguous arrays and transposes
+ *    tensors.
+ *  - Particle i < n_real is a real (integrated) particle, i >= n_real a fixed
+ *    boundary ("dummy") particle, in the order the reference keeps them
+ *    (Particles(1:ntotal), then ntotal+1:ntotal+ndummy).
+ *  - Every call returns 0 or a negative sphgpu_status; sphgpu_last_error()
+ *    gives the message. The Fortran shim turns a nonzero code into
+ *    `error stop`, as the reference does (asymmetricSearch.f90:38-39).
+ *  - One context = one HIP stream = one host thread. Calls are not
+ *    thread-safe on the same context.
+ */
+#ifndef SPHGPU_H
+#define SPHGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHGPU_ABI_VERSION 1
+#define SPHGPU_MAX_MATERIALS 8
+
+typedef enum sphgpu_status {
+    SPHGPU_OK = 0,
+    SPHGPU_E_OUT_OF_RANGE = -1,   /* particle outside the grid (asymmetricSearch.f90:38-39) */
+    SPHGPU_E_NONFINITE = -2,      /* NaN/Inf position, density or velocity */
+    SPHGPU_E_PAIR_CAPACITY = -3,  /* pair export buffer too small */
+    SPHGPU_E_BAD_CONFIG = -4,     /* unsupported switch combination */
+    SPHGPU_E_HIP = -5,            /* HIP runtime error */
+    SPHGPU_E_ARG = -6,            /* bad argument (NULL, size) */
+    SPHGPU_E_CAPACITY = -7        /* more particles than the context holds */
+} sphgpu_status;
+
+/* Arithmetic of the pair kernels. Positions, velocities, the neighbour test
+ * and the integrator stay FP64 in both modes. */
+typedef enum sphgpu_precision {
+    SPHGPU_FP64 = 0,
+    SPHGPU_MIXED = 1              /* FP32 pair physics, density stored as FP32 rho-rho0 */
+} sphgpu_precision;
+
+typedef enum sphgpu_eos {
+    SPHGPU_EOS_NONE = 0,          /* pressure left unchanged */
+    SPHGPU_EOS_TAIT = 1,          /* p = B((rho/rho0)^gamma - 1)   eos.f90:23-53 */
+    SPHGPU_EOS_LINEAR = 2,        /* p = K (rho - rho0)            eos.f90:149-157, 219-248 */
+    SPHGPU_EOS_ISOTHERMAL = 3     /* p = K rho                     eos.f90:57-64 */
+} sphgpu_eos;
+
+/* Material of the particles whose |Type| equals `type`. The reference
+ * hard-codes these per type and per case nick (eos.f90, viscosity.f90,
+ * timeIntegration.f90:149-166); here they are explicit. */
+typedef struct sphgpu_material {
+    int32_t type;        /* |Type| this entry applies to */
+    int32_t eos;         /* sphgpu_eos */
+    double rho0;         /* reference density */
+    double K;            /* Tait: B = c^2 rho0 / gamma; LINEAR: K/rho0; ISOTHERMAL: c^2 */
+    double gamma;        /* Tait exponent */
+    double eta;          /* viscosity.f90:14-33: dynamic viscosity, or shear modulus G for solids */
+    double sigma_y;      /* J2 cap (timeIntegration.f90:149-166); 0 = none */
+} sphgpu_material;
+
+typedef enum sphgpu_gravity_mode {
+    SPHGPU_GRAVITY_ALL = 0,       /* externalForce.f90:54-57 (default), :49-53 */
+    SPHGPU_GRAVITY_SKIP_SOLID = 1 /* externalForce.f90:42-48 (Type > 100 skipped) */
+} sphgpu_gravity_mode;
+
+/* Switches and constants of one run. Field names follow the reference's
+ * Config/Field globals (src/Initialize/controlDict.f90:16-74) and the
+ * constants its routines select by case nick. */
+typedef struct sphgpu_config {
+    int32_t dim;                  /* Field%Dim: 2 or 3 */
+    int32_t skf;                  /* Config%skf: 1 cubic spline, 2 Gauss, 3 quintic */
+    int32_t pa_sph;               /* Config%pa_sph: 2 (internalForce.f90:228-274) */
+    int32_t summation_density;    /* Config%sum_density_w: density.f90:10-80, else :83-107 */
+    int32_t normalize_density;    /* Config%norm_dens_w (Shepard, density.f90:25-76) */
+    int32_t viscosity;            /* Config%viscosity_w */
+    int32_t gravity;              /* Config%gravity_w */
+    int32_t gravity_mode;         /* sphgpu_gravity_mode */
+    int32_t external_force;       /* Config%ex_force_w (pair forces of externalForce.f90:103-174) */
+    int32_t artificial_viscosity; /* Config%arti_visc_w */
+    int32_t artificial_stress;    /* Config%arti_stress_w (2D only, artificialTerm.f90:37-96) */
+    int32_t kernel_correction;    /* Config%kernel_correciton_w (kernelGradientCorrection.f90) */
+    int32_t precision;            /* sphgpu_precision */
+    int32_t n_materials;
+    double gravity_g;             /* externalForce.f90:29: -9.81 on the last axis */
+    double av_alpha, av_beta, av_psi;            /* artificialViscosity.f90:28-79 */
+    double lj_factor, lj_r0, lj_p1, lj_p2;       /* externalForce.f90:62-101 (factor_s, r0, p1, p2) */
+    double pen_factor, pen_n1, pen_n2;           /* externalForce.f90:68-70 (factor_p, n1, n2) */
+    double couple_delta;                         /* externalForce.f90:72-97 (delta) */
+    double as_lambda, as_delta, as_exponent;     /* artificialTerm.f90:17-32 */
+    double dt;                                   /* Config%delta_t */
+    sphgpu_material mat[SPHGPU_MAX_MATERIALS];
+} sphgpu_config;
+
+/* Field selectors for sphgpu_download / sphgpu_upload_field. */
+typedef enum sphgpu_field {
+    SPHGPU_F_X = 0,       /* dim per particle */
+    SPHGPU_F_V = 1,       /* dim */
+    SPHGPU_F_RHO = 2,
+    SPHGPU_F_P = 3,
+    SPHGPU_F_E = 4,
+    SPHGPU_F_MASS = 5,
+    SPHGPU_F_H = 6,
+    SPHGPU_F_C = 7,
+    SPHGPU_F_DISP = 8,    /* dim */
+    SPHGPU_F_STRESS = 9,  /* dim*dim, Stress = Shear - p I (internalForce.f90:175-180) */
+    SPHGPU_F_SHEAR = 10,  /* dim*dim, deviatoric stress state of the integrator */
+    SPHGPU_F_DVDT = 11,   /* dim, Delta%Velocity */
+    SPHGPU_F_DRHODT = 12, /* Delta%Density */
+    SPHGPU_F_DEDT = 13,   /* Delta%Energy */
+    SPHGPU_F_DSDT = 14,   /* dim*dim */
+    SPHGPU_F_DIVX = 15,   /* divergencePosition */
+    SPHGPU_F_DIVV = 16,   /* divergenceVelocity */
+    SPHGPU_F_ETA = 17,    /* Viscosity */
+    SPHGPU_F_NFIELDS = 18
+} sphgpu_field;
+
+typedef struct sphgpu_ctx sphgpu_ctx;
+
+/* Library / ABI version, for a loader's sanity check. */
+int sphgpu_abi_version(void);
+
+/* Create a context on the current HIP device holding up to `capacity`
+ * particles (real + dummy). Replaces allocateParticleList
+ * (src/Initialize/SPH.f90:126-136) for the device copy of the state. */
+int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_ctx** out);
+
+/* Copy a particle set to the device (host keeps ownership). Any pointer
+ * except x, mass, rho, h, type may be NULL (zero). Resets the integrator
+ * (the next step is step 1, timeIntegration.f90:178-212). `shear` is the
+ * initial deviatoric stress (NULL = 0). */
+int sphgpu_upload(sphgpu_ctx* ctx, int64_t n_real, int64_t n_dummy,
+                  const double* x, const double* v, const double* mass,
+                  const double* rho, const double* p, const double* e,
+                  const double* c, const double* h, const double* shear,
+                  const int32_t* type, const int32_t* state,
+                  const int32_t* boundary);
+
+/* One evaluation of the derivatives at the resident state: BGGS neighbour
+ * set, density, EOS, internal/external/artificial forces and the fixed
+ * boundary, in the order of singleStep.f90:58-144. Like the reference it
+ * updates rho (summation density), p, stress, viscosity and the
+ * divergences in the resident state. Results stay on the device (read them
+ * with sphgpu_download). */
+int sphgpu_derivatives(sphgpu_ctx* ctx);
+
+/* Advance nsteps leap-frog steps on the device (timeIntegration.f90:
+ * 137-252); the host keeps output cadence. The first step after an upload
+ * is the reference's step 1. */
+int sphgpu_advance(sphgpu_ctx* ctx, int32_t nsteps);
+
+/* Copy one field of all n_real + n_dummy particles (in upload order) to
+ * host memory `dst` (doubles; count = n * components of the field). */
+int sphgpu_download(sphgpu_ctx* ctx, int32_t field, double* dst);
+
+/* Neighbour pairs of the current state, as BGGS builds them
+ * (asymmetricSearch.f90:160-257): for each real particle i, offsets[i] ..
+ * offsets[i+1]-1 index nbr[], the neighbour ids in ascending order.
+ * offsets has n_real+1 entries. If cap is too small, returns
+ * SPHGPU_E_PAIR_CAPACITY and sets *needed. */
+int sphgpu_export_pairs(sphgpu_ctx* ctx, int64_t* offsets, int32_t* nbr,
+                        int64_t cap, int64_t* needed);
+
+/* Neighbour statistics of the last neighbour search, as print_statistics
+ * reports them (src/NNPS/nnps.f90:377-416): out[0] total pairs, out[1] max
+ * and out[2] min neighbours per real particle, out[3] real particles with
+ * no neighbour. */
+int sphgpu_neighbour_stats(sphgpu_ctx* ctx, int64_t* out4);
+
+/* Geometry of the reference background grid for the current state:
+ * out[0..2] minCoor, out[3..5] maxCoor, out[6..8] cellNum (as doubles). */
+int sphgpu_grid_info(sphgpu_ctx* ctx, double* out9);
+
+/* Milliseconds spent in the last sphgpu_advance, measured with HIP events
+ * on the context's stream, and the average duration of the last call's
+ * force kernel launches. */
+int sphgpu_timing(sphgpu_ctx* ctx, double* advance_ms, double* force_kernel_ms,
+                  int64_t* force_launches);
+
+/* Number of the step the next sphgpu_advance performs (1 after upload). */
+int64_t sphgpu_step(sphgpu_ctx* ctx);
+
+const char* sphgpu_last_error(sphgpu_ctx* ctx);
+
+void sphgpu_destroy(sphgpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHGPU_H */

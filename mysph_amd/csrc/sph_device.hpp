@@ -1,0 +1,216 @@
+// Device-side data layout and per-particle / per-pair math of the SPH hot
+// path for gfx950 (MI355X). Included by sphgpu.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sphgpu.h"
+
+namespace sphgpu {
+
+constexpr double kPi = 3.141592653589793;                  // acos(-1.0_8), tools.f90:8
+constexpr double kGmrM1 = 0.2000000476837158203125;        // real(1.2_4,8) - 1, asymmetricSearch.f90:21
+constexpr int kNpg = 3;                                      // asymmetricSearch.f90:22
+constexpr int kBlock = 256;
+constexpr int kWin = 2;   // our cells are cutoff/kWin wide: a +-kWin cell sweep covers the cutoff
+constexpr int kNumXcd = 8;
+
+// error word bits (one word per context, OR-ed by kernels)
+constexpr int kErrOutOfRange = 1;
+constexpr int kErrNonFinite = 2;
+constexpr int kErrPairCap = 4;
+
+struct Material {
+    int32_t type, eos;
+    double rho0, K, gamma, eta, sigma_y;
+};
+
+// Constants of one context, passed by value to every kernel.
+struct Params {
+    int dim, skf, scale_k;
+    int n, n_real;
+    int nmat;
+    int summation_density, normalize_density, viscosity, gravity, gravity_mode;
+    int external_force, arti_visc, arti_stress, kgc, has_solid;
+    int mixed;          // FP32 pair physics with density stored as rho - rho0 in FP32
+    int pair_ext;       // external pair forces can act (wall range > 0 or several fluid types)
+    double g, av_alpha, av_beta, av_psi;
+    double lj_factor, lj_r0, lj_p1, lj_p2;
+    double pen_factor, pen_n1, pen_n2, couple_delta;
+    double as_lambda, as_delta, as_exponent;
+    double dt;
+    const Material* mat;   // device copy of the material table (nmat entries)
+    // our search grid (fixed per upload; keys are clamped into it)
+    double org[3];
+    double inv_cs;
+    int nc[3];
+    int64_t ncell;      // nc[0]*nc[1]*nc[2]; cell index ncell holds State == -1 particles
+    int nbr_cap;        // neighbour-list slots per particle
+    int64_t ldn;        // leading dimension of the neighbour list (particle capacity)
+};
+
+// Reference background grid of one step (asymmetricSearch.f90:49-131).
+struct RefGrid {
+    double minc[3], maxc[3];
+    int num[3];
+    int pad;
+};
+
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    // Work-groups are dispatched round-robin over the XCDs; remap so that each
+    // XCD walks a contiguous range of (spatially sorted) particles and its L2
+    // holds their neighbours.
+    int per = nb / kNumXcd, rem = nb % kNumXcd;
+    int x = b % kNumXcd, idx = b / kNumXcd;
+    return x * per + min(x, rem) + idx;
+}
+
+__device__ __forceinline__ const Material* find_mat(const Params& P, int type) {
+    int a = type < 0 ? -type : type;
+    for (int k = 0; k < P.nmat; ++k)
+        if (P.mat[k].type == a) return &P.mat[k];
+    return nullptr;
+}
+
+// eos.f90:23-53 (Tait), :149-157 and :219-248 (linear), :57-64 (isothermal).
+__device__ __forceinline__ double eos_p(const Material* m, double rho, double p_old) {
+    if (!m) return p_old;
+    switch (m->eos) {
+    case SPHGPU_EOS_TAIT: return m->K * (pow(rho / m->rho0, m->gamma) - 1.0);
+    case SPHGPU_EOS_LINEAR: return m->K * (rho - m->rho0);
+    case SPHGPU_EOS_ISOTHERMAL: return m->K * rho;
+    default: return p_old;
+    }
+}
+
+// Kernel normalisation factor alpha_d (kernel.f90:47-56, :89-92, :124-133).
+template <int DIM>
+__device__ __forceinline__ double kernel_factor(int skf, double h) {
+    if (skf == 1) {
+        if (DIM == 2) return 15.0 / (7.0 * kPi * (h * h));
+        return 3.0 / (2.0 * kPi * (h * h * h));
+    } else if (skf == 2) {
+        double sp = sqrt(kPi) * h;
+        return DIM == 2 ? 1.0 / (sp * sp) : 1.0 / (sp * sp * sp);
+    } else {
+        if (DIM == 2) return 7.0 / (478.0 * kPi * (h * h));
+        return 1.0 / (120.0 * kPi * (h * h * h));
+    }
+}
+
+// W and dW/dr-over-r: grad W = G * dx (kernel.f90:8-154).
+template <int DIM>
+__device__ __forceinline__ void kernel_wg(int skf, double r, double h, double factor,
+                                          double* w, double* G) {
+    double q = r / h;
+    if (skf == 1) {
+        if (q < 1.0) {
+            *w = factor * (2.0 / 3.0 - q * q + (q * q * q) / 2.0);
+            *G = factor * (-2.0 + 1.5 * q) / (h * h);
+        } else if (q <= 2.0) {
+            double t = 2.0 - q;
+            *w = factor * (1.0 / 6.0) * (t * t * t);
+            *G = factor * (-0.5) * (t * t) / h / r;
+        } else {
+            *w = 0.0;
+            *G = 0.0;
+        }
+    } else if (skf == 2) {
+        if (q < 3.0) {
+            *w = factor * exp(-q * q);
+            *G = *w * (-2.0 / (h * h));
+        } else {
+            *w = 0.0;
+            *G = 0.0;
+        }
+    } else {
+        double a = 3.0 - q, b = 2.0 - q, c = 1.0 - q;
+        double a2 = a * a, b2 = b * b, c2 = c * c;
+        if (q < 1.0) {
+            *w = factor * (a2 * a2 * a - 6.0 * (b2 * b2 * b) + 15.0 * (c2 * c2 * c));
+            *G = factor * (-120.0 + 120.0 * q - 50.0 * (q * q)) / (h * h);
+        } else if (q < 2.0) {
+            *w = factor * (a2 * a2 * a - 6.0 * (b2 * b2 * b));
+            *G = factor * (-5.0 * (a2 * a2) + 30.0 * (b2 * b2)) / h / r;
+        } else if (q <= 3.0) {
+            *w = factor * (a2 * a2 * a);
+            *G = factor * (-5.0 * (a2 * a2)) / h / r;
+        } else {
+            *w = 0.0;
+            *G = 0.0;
+        }
+    }
+}
+
+// Self contribution W(0, h) (density.f90:62).
+template <int DIM>
+__device__ __forceinline__ double kernel_w0(int skf, double h) {
+    double f = kernel_factor<DIM>(skf, h);
+    if (skf == 1) return f * (2.0 / 3.0);
+    if (skf == 2) return f;
+    return f * (243.0 - 6.0 * 32.0 + 15.0);
+}
+
+// Reference cell of a position, locate (asymmetricSearch.f90:27-47), with
+// correctly rounded operations in the reference's order (no contraction).
+__device__ __forceinline__ int ref_locate(double x, double mn, double mx, int num, bool* bad) {
+    if (!(x <= mx) || !(x >= mn)) {
+        *bad = true;
+        return 1;
+    }
+    double t = __ddiv_rn(__dsub_rn(x, mn), __dsub_rn(mx, mn));
+    t = __dadd_rn(__dmul_rn(t, (double)num), 1.0);
+    return (int)t;
+}
+
+__device__ __forceinline__ uint32_t pack_ref(int c0, int c1, int c2) {
+    return (uint32_t)(c0 - 1) | ((uint32_t)(c1 - 1) << 10) | ((uint32_t)(c2 - 1) << 20);
+}
+
+__device__ __forceinline__ int ref_axis(uint32_t pk, int d) { return (int)((pk >> (10 * d)) & 1023u); }
+
+// Solve A x = b in place of b by the column-pivot elimination of MCPE
+// (tools.f90:287-354), including its early return on a singular row.
+template <int DIM>
+__device__ __forceinline__ void mcpe_solve(const double* Ain, double* b) {
+    double A[DIM * DIM], B[DIM], X[DIM];
+#pragma unroll
+    for (int q = 0; q < DIM * DIM; ++q) A[q] = Ain[q];
+#pragma unroll
+    for (int q = 0; q < DIM; ++q) { B[q] = b[q]; X[q] = b[q]; }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        int p = i;
+        for (int j = i + 1; j < DIM; ++j)
+            if (fabs(A[p * DIM + i]) < fabs(A[j * DIM + i])) p = j;
+        for (int j = 0; j < DIM; ++j) {
+            double t = A[i * DIM + j]; A[i * DIM + j] = A[p * DIM + j]; A[p * DIM + j] = t;
+        }
+        double t = B[i]; B[i] = B[p]; B[p] = t;
+    }
+    for (int i = 0; i < DIM - 1; ++i)
+        for (int j = i + 1; j < DIM; ++j) {
+            double f = A[j * DIM + i] / A[i * DIM + i];
+            double s = 0.0;
+            for (int k = i; k < DIM; ++k) {
+                A[j * DIM + k] = A[j * DIM + k] - A[i * DIM + k] * f;
+                s = s + A[j * DIM + k] * A[j * DIM + k];
+            }
+            B[j] = B[j] - B[i] * f;
+            if (s == 0.0) return;  // b keeps its input value
+        }
+    X[DIM - 1] = B[DIM - 1] / A[(DIM - 1) * DIM + DIM - 1];
+    for (int i = DIM - 2; i >= 0; --i) {
+        for (int j = DIM - 1; j >= i + 1; --j) B[i] = B[i] - A[i * DIM + j] * X[j];
+        X[i] = B[i] / A[i * DIM + i];
+    }
+#pragma unroll
+    for (int q = 0; q < DIM; ++q) b[q] = X[q];
+}
+
+__device__ __forceinline__ double comp(const double4& v, int d) {
+    return d == 0 ? v.x : (d == 1 ? v.y : v.z);
+}
+
+}  // namespace sphgpu

@@ -1,0 +1,1775 @@
+// libsphgpu: the reference SPH step (NNPS + pair physics + leap-frog update)
+// as HIP kernels for gfx950. C ABI in include/sphgpu.h.
+//
+// Step pipeline on one stream (every array stays resident on the device):
+//   k_grid         reference BGGS grid of the step (asymmetricSearch.f90:49-131)
+//   k_keys         our cell key (clamped fixed grid) + reference cell per particle,
+//                  cell counts
+//   k_scan*        exclusive scan of the counts -> cell start table
+//   k_scatter      particles into cell order; k_cellsort makes the order
+//                  within a cell deterministic (ascending slot)
+//   k_point_build  leap-frog half-step predict + EOS (timeIntegration.f90:137-170,
+//                  internalForce.f90:103-134) and the neighbour search with the
+//                  reference's acceptance rule and window (asymmetricSearch.f90:206-254)
+//   k_kgc / k_sumdens / k_strain / k_solid_point   optional passes
+//   k_force        continuity, divergence, pressure/viscous/solid stress, wall and
+//                  coupling forces, artificial viscosity and stress, gravity
+//   k_correct      leap-frog correct (timeIntegration.f90:178-252) + bounding box
+//                  partials for the next step's grid
+// All sums are gathers (one writer per particle, no floating-point atomics).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "sph_device.hpp"
+
+using namespace sphgpu;
+
+// ----------------------------------------------------------------- device
+
+struct Dev {
+    double4* posh;
+    double4* vm;
+    double* rho;
+    double* e;
+    double* c;
+    double* p;
+    int4* meta;
+    double4* disp;
+    double4* dv;      // dvdt xyz, drhodt w
+    double* de;
+    double4* vprev;   // v prev xyz, rho prev w
+    double* eprev;
+    double4* th;      // rho, p/rho^2, c, m/rho
+    double2* div;     // divergencePosition, divergenceVelocity
+    double* eta;
+    double* tds;      // viscous / elastic heating Tds/dt
+    double* Ep;       // eta * edot / rho^2 (dim*dim), read by neighbours' viscous term
+    double* Ed;       // solid strain rate edot (dim*dim)
+    double* rot;      // rotation rate (solid, dim*dim)
+    double* shear;    // S
+    double* shear_prev;
+    double* dsdt;
+    double* stress;   // S - p I
+    double* Sp;       // stress / rho^2
+    double* asR;      // artificial-stress tensor R (2D solids, 4)
+    double* kgcA;     // kernel-gradient-correction matrix A (dim*dim)
+    double* rho_new;  // summation density result
+    uint32_t* key;
+    uint32_t* refcell;
+    int* perm;
+    int* count;
+    int* start;
+    int* fill;
+    int* bsum;
+    int* nbr;
+    int* ncnt;
+    double* partial;
+    RefGrid* grid;
+    int* err;
+    int* maxcnt;
+    Material* mat;
+    // mixed precision: FP32 copies read by the pair kernels, and FP32 density
+    // deviation / derivatives (positions and velocities stay FP64)
+    float4* vmf;      // vx, vy, vz, mass
+    float4* thf;      // rho, p/rho^2, c, m/rho
+    float* drho;      // rho - rho0
+    float* drho_prev;
+    float4* dvf;      // dvdt xyz, d(rho)/dt w
+    float* def;       // de/dt
+};
+
+// Block min/max of positions -> partial[blockIdx*6 + (0..2 min, 3..5 max)].
+__device__ __forceinline__ void bbox_block_reduce(double mn[3], double mx[3], double* partial,
+                                                  int blk) {
+    __shared__ double smn[3][kBlock / 64];
+    __shared__ double smx[3][kBlock / 64];
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double a = mn[d], b = mx[d];
+        for (int o = 32; o > 0; o >>= 1) {
+            a = fmin(a, __shfl_xor(a, o));
+            b = fmax(b, __shfl_xor(b, o));
+        }
+        if (lane == 0) { smn[d][wid] = a; smx[d][wid] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int d = threadIdx.x;
+        double a = smn[d][0], b = smx[d][0];
+        for (int w = 1; w < kBlock / 64; ++w) { a = fmin(a, smn[d][w]); b = fmax(b, smx[d][w]); }
+        partial[blk * 6 + d] = a;
+        partial[blk * 6 + 3 + d] = b;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bbox(Params P, Dev D) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (s < P.n) {
+        double4 q = D.posh[s];
+        mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
+        if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z)) atomicOr(D.err, kErrNonFinite);
+    }
+    bbox_block_reduce(mn, mx, D.partial, blockIdx.x);
+}
+
+// Reference grid of this step, asymmetricSearch.f90:49-131. One block.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_grid(Params P, Dev D, int nparts) {
+    __shared__ double smn[3][kBlock], smx[3][kBlock];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nparts; b += kBlock)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], D.partial[b * 6 + d]);
+            mx[d] = fmax(mx[d], D.partial[b * 6 + 3 + d]);
+        }
+    for (int d = 0; d < 3; ++d) { smn[d][threadIdx.x] = mn[d]; smx[d][threadIdx.x] = mx[d]; }
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int d = 0; d < 3; ++d) {
+                smn[d][threadIdx.x] = fmin(smn[d][threadIdx.x], smn[d][threadIdx.x + o]);
+                smx[d][threadIdx.x] = fmax(smx[d][threadIdx.x], smx[d][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        RefGrid g;
+        double delta[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; ++d) { g.minc[d] = 0.0; g.maxc[d] = 0.0; g.num[d] = 1; }
+        for (int d = 0; d < DIM; ++d) {
+            delta[d] = __dsub_rn(smx[d][0], smn[d][0]);
+            g.maxc[d] = __dadd_rn(smx[d][0], __dmul_rn(kGmrM1, delta[d]));
+            g.minc[d] = __dsub_rn(smn[d][0], __dmul_rn(kGmrM1, delta[d]));
+        }
+        double N = (double)P.n;
+        if (DIM == 2) {
+            double t = __dsqrt_rn(__ddiv_rn(__dmul_rn(N, delta[0]), __dmul_rn(delta[1], (double)kNpg)));
+            int c1 = (t >= 0.0 && t < 1e9) ? (int)t + 1 : 0;   // pow(t, 1/2) correctly rounded
+            g.num[0] = min(c1, 1000);
+            double t2 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[1]), delta[0]);
+            int c2 = (t2 >= 0.0 && t2 < 1e9) ? (int)t2 + 1 : 0;
+            g.num[1] = min(c2, 1000);
+        } else {
+            double t = __ddiv_rn(__dmul_rn(__dmul_rn(N, delta[0]), delta[0]),
+                                 __dmul_rn(__dmul_rn(delta[1], delta[2]), (double)kNpg));
+            t = pow(t, 1.0 / 3.0);
+            int c1 = (t >= 0.0 && t < 1e9) ? (int)t + 1 : 0;
+            g.num[0] = min(c1, 1000);
+            double t2 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[1]), delta[0]);
+            double t3 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[2]), delta[0]);
+            int c2 = (t2 >= 0.0 && t2 < 1e9) ? (int)t2 + 1 : 0;
+            int c3 = (t3 >= 0.0 && t3 < 1e9) ? (int)t3 + 1 : 0;
+            g.num[1] = min(c2, 1000);
+            g.num[2] = min(c3, 1000);
+        }
+        // a flat bounding box divides by zero above (undefined in the
+        // reference): report it and keep the grid usable
+        for (int d = 0; d < DIM; ++d)
+            if (!(delta[d] > 0.0) || g.num[d] < 1) {
+                atomicOr(D.err, kErrOutOfRange);
+                g.num[d] = 1;
+                g.maxc[d] = g.minc[d] + 1.0;
+            }
+        g.pad = 0;
+        *D.grid = g;
+    }
+}
+
+// Our cell key (clamped into the fixed grid) and the reference cell.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    double4 q = D.posh[s];
+    int4 mt = D.meta[s];
+    uint32_t key;
+    if (mt.y == -1) {
+        key = (uint32_t)P.ncell;
+    } else {
+        double xs[3] = {q.x, q.y, q.z};
+        int c[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; ++d) {
+            double t = floor((xs[d] - P.org[d]) * P.inv_cs);
+            t = fmin(fmax(t, 0.0), (double)(P.nc[d] - 1));
+            c[d] = (int)t;
+        }
+        key = (uint32_t)((int64_t)c[0] + (int64_t)P.nc[0] * ((int64_t)c[1] + (int64_t)P.nc[1] * c[2]));
+        const RefGrid g = *D.grid;
+        bool bad = false;
+        int rc[3] = {1, 1, 1};
+        for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
+        if (bad) atomicOr(D.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+                                                                                  : kErrNonFinite);
+        D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
+    }
+    D.key[s] = key;
+    atomicAdd(&D.count[key], 1);
+}
+
+// Exclusive scan of count[0..m) into start[0..m], 2048 entries per block.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+    __shared__ int ws[kBlock / 64];
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wid] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) off += ws[w];
+        tot += ws[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan1(const int* __restrict__ count, int m, int* bsum) {
+    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < m) s += count[base + k];
+    int tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb) {
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kBlock) {
+        int b = b0 + threadIdx.x;
+        int v = b < nb ? bsum[b] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, &tot);
+        if (b < nb) bsum[b] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan3(int* count, int m, const int* bsum, int* start,
+                                                  int* fill, int total) {
+    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < m) ? count[base + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_exclusive_scan(s, &tot) + bsum[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < m) {
+            start[base + k] = ex;
+            fill[base + k] = ex;
+            count[base + k] = 0;
+        }
+        ex += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[m] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int pos = atomicAdd(&D.fill[D.key[s]], 1);
+    D.perm[pos] = s;
+}
+
+// Ascending slot order inside each cell: the pair order, and so every sum,
+// is the same from run to run.
+__global__ __launch_bounds__(kBlock) void k_cellsort(const int* __restrict__ start, int* perm,
+                                                     int m) {
+    int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m) return;
+    int a = start[c], b = start[c + 1];
+    for (int i = a + 1; i < b; ++i) {
+        int v = perm[i];
+        int j = i - 1;
+        while (j >= a && perm[j] > v) {
+            perm[j + 1] = perm[j];
+            --j;
+        }
+        perm[j + 1] = v;
+    }
+}
+
+// Leap-frog predict of one real particle (timeIntegration.f90:137-170).
+template <int DIM>
+__device__ __forceinline__ void predict_one(const Params& P, const Dev& D, int s, int4 mt,
+                                            bool first) {
+    double4 v = D.vm[s];
+    double rho = D.rho[s], e = D.e[s];
+    D.vprev[s] = make_double4(v.x, v.y, v.z, rho);
+    D.eprev[s] = e;
+    if (first) return;
+    double h2 = P.dt / 2;
+    double4 dv = D.dv[s];
+    e = e + h2 * D.de[s];
+    if (e < 0) e = 0;
+    if (!P.summation_density) rho = rho + h2 * dv.w;
+    v.x = v.x + h2 * dv.x;
+    v.y = v.y + h2 * dv.y;
+    if (DIM == 3) v.z = v.z + h2 * dv.z;
+    D.vm[s] = v;
+    D.rho[s] = rho;
+    D.e[s] = e;
+    if (P.has_solid && mt.x > 100) {
+        const Material* m = find_mat(P, mt.x);
+        if (!m) return;
+        double* S = D.shear + (int64_t)s * DIM * DIM;
+        double* Sp = D.shear_prev + (int64_t)s * DIM * DIM;
+        const double* dS = D.dsdt + (int64_t)s * DIM * DIM;
+        double J2 = 0.0;
+        for (int q = 0; q < DIM * DIM; ++q) {
+            Sp[q] = S[q];
+            S[q] = S[q] + h2 * dS[q];
+        }
+        if (m->sigma_y > 0) {
+            for (int q = 0; q < DIM * DIM; ++q) J2 += S[q] * S[q];
+            double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
+            for (int q = 0; q < DIM * DIM; ++q) S[q] = S[q] * sc;
+        }
+    }
+}
+
+// EOS and the per-particle pair inputs th = (rho, p/rho^2, c, m/rho).
+__device__ __forceinline__ void eos_th(const Params& P, const Dev& D, int s, int4 mt) {
+    double rho = D.rho[s];
+    double p = D.p[s];
+    if (!(mt.x > 100 || mt.x < -100)) p = eos_p(find_mat(P, mt.x), rho, p);
+    D.p[s] = p;
+    double m = D.vm[s].w;
+    D.th[s] = make_double4(rho, p / (rho * rho), D.c[s], m / rho);
+}
+
+// Mixed precision: predict with FP64 velocity/position accumulation and the
+// FP32 density deviation drho = rho - rho0.
+template <int DIM>
+__device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int s, bool first) {
+    double4 v = D.vm[s];
+    float dr = D.drho[s];
+    double e = D.e[s];
+    D.vprev[s] = make_double4(v.x, v.y, v.z, 0.0);
+    D.drho_prev[s] = dr;
+    D.eprev[s] = e;
+    if (first) return;
+    const double h2 = P.dt / 2;
+    float4 dv = D.dvf[s];
+    e = e + h2 * (double)D.def[s];
+    if (e < 0) e = 0;
+    dr = dr + (float)h2 * dv.w;
+    v.x = v.x + h2 * (double)dv.x;
+    v.y = v.y + h2 * (double)dv.y;
+    if (DIM == 3) v.z = v.z + h2 * (double)dv.z;
+    D.vm[s] = v;
+    D.drho[s] = dr;
+    D.e[s] = e;
+}
+
+// FP32 EOS on the density deviation: the Tait term (rho/rho0)^gamma - 1 is
+// evaluated as a polynomial in u = drho/rho0 (gamma = 7) or expm1(gamma
+// log1p(u)), so it does not cancel.
+__device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, int type) {
+    const Material* m = find_mat(P, type);
+    float dr = D.drho[s];
+    float rho0 = m ? (float)m->rho0 : 0.0f;
+    float rho = rho0 + dr;
+    float p;
+    if (m && m->eos == SPHGPU_EOS_TAIT) {
+        float u = dr / rho0;
+        float B = (float)m->K;
+        if (m->gamma == 7.0)
+            p = B * u * (7.0f + u * (21.0f + u * (35.0f + u * (35.0f + u * (21.0f + u * (7.0f + u))))));
+        else
+            p = B * expm1f((float)m->gamma * log1pf(u));
+    } else if (m && m->eos == SPHGPU_EOS_LINEAR) {
+        p = (float)m->K * dr;
+    } else if (m && m->eos == SPHGPU_EOS_ISOTHERMAL) {
+        p = (float)m->K * rho;
+    } else {
+        p = (float)D.p[s];
+    }
+    D.p[s] = (double)p;
+    double4 v = D.vm[s];
+    float mass = (float)v.w;
+    float irho = 1.0f / rho;
+    D.thf[s] = make_float4(rho, p * irho * irho, (float)D.c[s], mass * irho);
+    D.vmf[s] = make_float4((float)v.x, (float)v.y, (float)v.z, mass);
+}
+
+// Predict + EOS for every particle, then the neighbour search for real
+// particles: candidates from the +-kWin cell sweep of our grid, accepted with
+// the reference rule r < scale_k * h_i (r from the sum of squares in axis order,
+// correctly rounded, no contraction), j != i, and j inside i's reference window
+// (asymmetricSearch.f90:209-237).
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mode) {
+    // mode bit 0: first step (no predict); bit 1: skip predict entirely
+    // (derivatives only); bit 2: count only (no list writes)
+    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    bool real = mt.w < P.n_real;
+    if (MIXED) {
+        if (real && !(mode & 2)) predict_mixed<DIM>(P, D, s, mode & 1);
+        eos_mixed(P, D, s, mt.x);
+    } else {
+        if (real && !(mode & 2)) predict_one<DIM>(P, D, s, mt, mode & 1);
+        if (!P.summation_density) eos_th(P, D, s, mt);
+    }
+    if (!real || mt.y == -1) {
+        if (real) D.ncnt[s] = 0;
+        return;
+    }
+    double4 qi = D.posh[s];
+    double hi = qi.w;
+    double cut = hi * (double)P.scale_k;
+    double cut2 = cut * cut * (1.0 + 1e-12);
+    uint32_t key = D.key[s];
+    int cx = (int)(key % (uint32_t)P.nc[0]);
+    int rest = (int)(key / (uint32_t)P.nc[0]);
+    int cy = rest % P.nc[1];
+    int cz = rest / P.nc[1];
+    const RefGrid g = *D.grid;
+    uint32_t rci = D.refcell[s];
+    int wr[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
+    for (int d = 0; d < DIM; ++d) {
+        // cellNumPerHSML, asymmetricSearch.f90:209-211
+        wr[d] = (int)(hi / (g.maxc[d] - g.minc[d]) * g.num[d]) + 1;
+        rc[d] = ref_axis(rci, d);
+    }
+    int cnt = 0;
+    const int cap = (mode & 4) ? 0 : P.nbr_cap;
+    const int zlo = DIM == 3 ? max(cz - kWin, 0) : 0, zhi = DIM == 3 ? min(cz + kWin, P.nc[2] - 1) : 0;
+    const int ylo = max(cy - kWin, 0), yhi = min(cy + kWin, P.nc[1] - 1);
+    const int xlo = max(cx - kWin, 0), xhi = min(cx + kWin, P.nc[0] - 1);
+    for (int z = zlo; z <= zhi; ++z)
+        for (int y = ylo; y <= yhi; ++y) {
+            int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
+            int kb = D.start[row + xlo], ke = D.start[row + xhi + 1];
+            for (int k = kb; k < ke; ++k) {
+                int j = D.perm[k];
+                double4 qj = D.posh[j];
+                double dx0 = __dsub_rn(qi.x, qj.x);
+                double dx1 = __dsub_rn(qi.y, qj.y);
+                double dr = __dadd_rn(__dmul_rn(dx0, dx0), __dmul_rn(dx1, dx1));
+                if (DIM == 3) {
+                    double dx2 = __dsub_rn(qi.z, qj.z);
+                    dr = __dadd_rn(dr, __dmul_rn(dx2, dx2));
+                }
+                if (!(dr < cut2) || j == s) continue;
+                if (!(__dsqrt_rn(dr) < cut)) continue;
+                uint32_t rcj = D.refcell[j];
+                bool inwin = true;
+                for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
+                if (!inwin) continue;
+                if (cnt < cap) D.nbr[(int64_t)cnt * P.ldn + s] = j;
+                ++cnt;
+            }
+        }
+    D.ncnt[s] = cnt;
+    if (cnt > P.nbr_cap && !(mode & 4)) atomicOr(D.err, kErrPairCap);
+    atomicMax(D.maxcnt, cnt);
+}
+
+// Pair geometry of target i and neighbour j: dx = x_i - x_j, r, W, grad W.
+template <int DIM>
+struct PairGeo {
+    double dx[3];
+    double r2, r, w, gw[3];
+};
+
+template <int DIM>
+__device__ __forceinline__ void pair_geo(const Params& P, const double4& qi, const double4& qj,
+                                         double factor, const double* A, PairGeo<DIM>& g) {
+    g.dx[0] = qi.x - qj.x;
+    g.dx[1] = qi.y - qj.y;
+    g.dx[2] = DIM == 3 ? qi.z - qj.z : 0.0;
+    double dr = g.dx[0] * g.dx[0] + g.dx[1] * g.dx[1];
+    if (DIM == 3) dr += g.dx[2] * g.dx[2];
+    g.r2 = dr;
+    g.r = sqrt(dr);
+    double G;
+    kernel_wg<DIM>(P.skf, g.r, qi.w, factor, &g.w, &G);
+    for (int d = 0; d < 3; ++d) g.gw[d] = (d < DIM) ? G * g.dx[d] : 0.0;
+    if (A) mcpe_solve<DIM>(A, g.gw);
+}
+
+// kernelGradientCorrection.f90:20-35: A_i = sum_j V_j (x_j - x_i)(e) dW(d).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
+    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    if (mt.w >= P.n_real || mt.y == -1) return;
+    double A[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) A[q] = 0.0;
+    if (mt.x > 100) {
+        double4 qi = D.posh[s];
+        double f = kernel_factor<DIM>(P.skf, qi.w);
+        int n = min(D.ncnt[s], P.nbr_cap);
+        int ai = mt.x < 0 ? -mt.x : mt.x;
+        for (int k = 0; k < n; ++k) {
+            int j = D.nbr[(int64_t)k * P.ldn + s];
+            int tj = D.meta[j].x;
+            int aj = tj < 0 ? -tj : tj;
+            if ((ai - 100) * (aj - 100) < 0) continue;
+            double4 qj = D.posh[j];
+            PairGeo<DIM> g;
+            pair_geo<DIM>(P, qi, qj, f, nullptr, g);
+            double V = D.vm[j].w / D.rho[j];
+            for (int d = 0; d < DIM; ++d)
+                for (int e = 0; e < DIM; ++e) A[d * DIM + e] += (-g.dx[e]) * g.gw[d] * V;
+        }
+    }
+    for (int q = 0; q < DIM * DIM; ++q) D.kgcA[(int64_t)s * DIM * DIM + q] = A[q];
+}
+
+// density.f90:10-80 (summation density, optional Shepard normalisation).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_sumdens(Params P, Dev D) {
+    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    double4 qi = D.posh[s];
+    double f = kernel_factor<DIM>(P.skf, qi.w);
+    double w0 = kernel_w0<DIM>(P.skf, qi.w);
+    double mi = D.vm[s].w;
+    double rho = mi * w0;
+    double wi = 0.0;
+    bool norm = P.normalize_density && D.div[s].x < 1.5;
+    if (norm) wi = mi / D.rho[s] * w0;
+    int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
+    for (int k = 0; k < n; ++k) {
+        int j = D.nbr[(int64_t)k * P.ldn + s];
+        double4 qj = D.posh[j];
+        PairGeo<DIM> g;
+        pair_geo<DIM>(P, qi, qj, f, nullptr, g);
+        double mj = D.vm[j].w;
+        rho += mj * g.w;
+        if (norm) wi += mj / D.rho[j] * g.w;
+    }
+    if (norm) rho = rho / wi;
+    D.rho_new[s] = rho;
+}
+
+__global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    if (mt.w < P.n_real) D.rho[s] = D.rho_new[s];
+    eos_th(P, D, s, mt);
+}
+
+// Strain-rate (and solid rotation-rate) sums, internalForce.f90:48-100, and the
+// fluid viscous heating tdsdt (:108-112). T(d,e) = sum_j V_j (v_j - v_i)_d dW_e.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
+    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    const Material* m = find_mat(P, mt.x);
+    double eta = (P.viscosity && m) ? m->eta : 0.0;   // viscosity.f90:14-33
+    D.eta[s] = eta;
+    double T[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) T[q] = 0.0;
+    bool real = mt.w < P.n_real && mt.y != -1;
+    if (real) {
+        double4 qi = D.posh[s];
+        double4 vi = D.vm[s];
+        double f = kernel_factor<DIM>(P.skf, qi.w);
+        const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
+        int n = min(D.ncnt[s], P.nbr_cap);
+        for (int k = 0; k < n; ++k) {
+            int j = D.nbr[(int64_t)k * P.ldn + s];
+            double4 qj = D.posh[j];
+            double4 vj = D.vm[j];
+            PairGeo<DIM> g;
+            pair_geo<DIM>(P, qi, qj, f, A, g);
+            double V = vj.w / D.th[j].x;
+            double dvn[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+            for (int d = 0; d < DIM; ++d)
+                for (int e = 0; e < DIM; ++e) T[d * DIM + e] += V * dvn[d] * g.gw[e];
+        }
+    }
+    int ai = mt.x < 0 ? -mt.x : mt.x;
+    double rho = D.th[s].x;
+    int64_t o = (int64_t)s * DIM * DIM;
+    if (ai <= 100) {
+        double tr = 0.0;
+        for (int d = 0; d < DIM; ++d) tr += T[d * DIM + d];
+        double sq = 0.0;
+        for (int d = 0; d < DIM; ++d)
+            for (int e = 0; e < DIM; ++e) {
+                double E = 2.0 * T[d * DIM + e] - (d == e ? (2.0 / 3.0) * tr : 0.0);
+                sq += E * E;
+                D.Ep[o + d * DIM + e] = eta * E / (rho * rho);
+            }
+        D.tds[s] = 0.5 * eta / rho * sq;
+    } else {
+        for (int d = 0; d < DIM; ++d)
+            for (int e = 0; e < DIM; ++e) {
+                double E = 0.5 * (T[d * DIM + e] + T[e * DIM + d]);
+                D.Ed[o + d * DIM + e] = E;                                           // edot
+                D.rot[o + d * DIM + e] = 0.5 * (T[d * DIM + e] - T[e * DIM + d]);  // rdot
+                D.Ep[o + d * DIM + e] = eta * E / (rho * rho);
+            }
+    }
+}
+
+// Solid point pass, internalForce.f90:137-180 and artificialTerm.f90:42-69.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_solid_point(Params P, Dev D) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    int ai = mt.x < 0 ? -mt.x : mt.x;
+    if (ai <= 100) return;
+    const Material* m = find_mat(P, mt.x);
+    int64_t o = (int64_t)s * DIM * DIM;
+    double E[DIM * DIM], R[DIM * DIM], S[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) {
+        E[q] = P.viscosity ? D.Ed[o + q] : 0.0;
+        R[q] = P.viscosity ? D.rot[o + q] : 0.0;
+        S[q] = D.shear[o + q];
+    }
+    double aux = 0.0;
+    for (int d = 0; d < DIM; ++d) aux += E[d * DIM + d];
+    aux = aux / 3;
+    for (int d = 0; d < DIM; ++d) E[d * DIM + d] -= aux;
+    double4 t = D.th[s];
+    double rho = t.x;
+    double sq = 0.0;
+    for (int q = 0; q < DIM * DIM; ++q) sq += S[q] * E[q];
+    D.tds[s] = sq / rho;
+    double p = eos_p(m, rho, D.p[s]);
+    D.p[s] = p;
+    double eta = (P.viscosity && m) ? m->eta : 0.0;
+    for (int d = 0; d < DIM; ++d)
+        for (int dd = 0; dd < DIM; ++dd) {
+            double v = 2 * eta * E[d * DIM + dd];
+            for (int k = 0; k < DIM; ++k)
+                v = v + S[d * DIM + k] * R[dd * DIM + k] + S[dd * DIM + k] * R[d * DIM + k];
+            if (mt.w < P.n_real) D.dsdt[o + d * DIM + dd] = v;
+        }
+    double sig[DIM * DIM];
+    for (int d = 0; d < DIM; ++d)
+        for (int e = 0; e < DIM; ++e) {
+            sig[d * DIM + e] = S[d * DIM + e] - (d == e ? p : 0.0);
+            D.stress[o + d * DIM + e] = sig[d * DIM + e];
+            D.Sp[o + d * DIM + e] = sig[d * DIM + e] / (rho * rho);
+        }
+    D.th[s] = make_double4(rho, p / (rho * rho), t.z, t.w);
+    if (DIM == 2 && P.arti_stress && mt.x >= 100) {
+        double a = sig[0] - sig[3];
+        double theta = (a == 0.0) ? 0.5 * kPi : 0.5 * atan((sig[1] + sig[2]) / a);
+        double sn = sin(theta), cs = cos(theta);
+        double ps0 = cs * cs * sig[0] + sn * sn * sig[3] + 2 * sn * cs * sig[1];
+        double ps1 = sn * sn * sig[0] + cs * cs * sig[3] - 2 * sn * cs * sig[1];
+        double T0 = ps0 > 0 ? -P.as_lambda * ps0 / (rho * rho) : 0.0;
+        double T1 = ps1 > 0 ? -P.as_lambda * ps1 / (rho * rho) : 0.0;
+        double* Rs = D.asR + (int64_t)s * 4;
+        Rs[0] = cs * cs * T0 + sn * sn * T1;
+        Rs[3] = sn * sn * T0 + cs * cs * T1;
+        Rs[1] = sn * cs * (T0 - T1);
+        Rs[2] = Rs[1];
+    }
+}
+
+// The force pass: everything single_step adds into Delta (singleStep.f90:139-144)
+// plus continuity density and the divergences.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
+    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    const int ti = mt.x;
+    const int ai = ti < 0 ? -ti : ti;
+    const bool fluid = ai <= 100;
+    double4 qi = D.posh[s];
+    double4 vi = D.vm[s];
+    double4 thi = D.th[s];
+    const double f = kernel_factor<DIM>(P.skf, qi.w);
+    const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
+    double Ei[DIM * DIM], Si[DIM * DIM], Ri[4] = {0, 0, 0, 0};
+    const bool visc = P.viscosity && fluid;
+    for (int q = 0; q < DIM * DIM; ++q) {
+        Ei[q] = visc ? D.Ep[(int64_t)s * DIM * DIM + q] : 0.0;
+        Si[q] = (!fluid) ? D.Sp[(int64_t)s * DIM * DIM + q] : 0.0;
+    }
+    const bool as_on = DIM == 2 && P.arti_stress && ti >= 100;
+    double wdelta = 1.0;
+    if (P.arti_stress && DIM == 2) {
+        for (int q = 0; q < 4; ++q) Ri[q] = (ti >= 100) ? D.asR[(int64_t)s * 4 + q] : 0.0;
+        double G;
+        kernel_wg<DIM>(P.skf, P.as_delta, qi.w, f, &wdelta, &G);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    double de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0;
+    int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
+    for (int k = 0; k < n; ++k) {
+        int j = D.nbr[(int64_t)k * P.ldn + s];
+        double4 qj = D.posh[j];
+        double4 vj = D.vm[j];
+        double4 thj = D.th[j];
+        PairGeo<DIM> g;
+        pair_geo<DIM>(P, qi, qj, f, A, g);
+        const double mj = vj.w;
+        double dv[3] = {vi.x - vj.x, vi.y - vj.y, DIM == 3 ? vi.z - vj.z : 0.0};
+        double dvg = dv[0] * g.gw[0] + dv[1] * g.gw[1] + (DIM == 3 ? dv[2] * g.gw[2] : 0.0);
+        double dxg = g.dx[0] * g.gw[0] + g.dx[1] * g.gw[1] + (DIM == 3 ? g.dx[2] * g.gw[2] : 0.0);
+        // continuity (density.f90:98-101) and divergences (divergence.f90:17-27)
+        sdiv_v += thj.w * dvg;
+        sdiv_x += thj.w * dxg;
+        // internal force, SPH = 2 (internalForce.f90:228-274)
+        double aux = thi.y + thj.y;
+        if (fluid) {
+            for (int d = 0; d < DIM; ++d) {
+                double mv = 0.0;
+                if (visc) {
+                    const double* Ej = D.Ep + (int64_t)j * DIM * DIM;
+                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + Ej[d * DIM + e]) * g.gw[e];
+                }
+                acc[d] += mj * (-aux * g.gw[d] + mv);
+            }
+        } else {
+            const double* Sj = D.Sp + (int64_t)j * DIM * DIM;
+            for (int d = 0; d < DIM; ++d) {
+                double sv = 0.0;
+                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + Sj[d * DIM + e]) * g.gw[e];
+                acc[d] += mj * sv;
+            }
+        }
+        de_in += mj * aux * dvg;
+        // external pair forces (externalForce.f90:103-174)
+        if (P.external_force) {
+            const int tj = D.meta[j].x;
+            if (ti > 0 && tj < 0 && g.r < P.lj_r0) {
+                double rr = P.lj_r0 / g.r;
+                double fl = P.lj_factor * (pow(rr, P.lj_p1) - pow(rr, P.lj_p2)) / (g.r * g.r);
+                for (int d = 0; d < DIM; ++d) acc[d] += fl * g.dx[d];
+            }
+            if (ti != tj && ti > 0 && tj > 0) {
+                if (ti < 100 && tj < 100) {
+                    double pe = (qi.w + qj.w) / (2 * g.r);
+                    if (pe >= 1) {
+                        double fp = P.pen_factor * (pow(pe, P.pen_n1) - pow(pe, P.pen_n2)) / (g.r * g.r);
+                        for (int d = 0; d < DIM; ++d) acc[d] += fp * g.dx[d];
+                    }
+                } else if (!(ti > 100 && tj > 100)) {
+                    double et = g.r / 0.75 / ((qi.w + qj.w) * 0.5);
+                    double chi = 1 - g.r / P.couple_delta;
+                    if (chi < 0 || chi > 1) chi = 0;
+                    double fc = 0;
+                    if (et <= 0 || et >= 2) fc = 0;
+                    else if (et <= 2.0 / 3) fc = 2.0 / 3;
+                    else if (et <= 1) fc = 2 * et - 1.5 * et * et;
+                    else if (et < 2) fc = 0.5 * ((2 - et) * (2 - et));
+                    double fcp = (double)0.01f * (thi.z * thi.z) * chi * fc / (g.r * g.r);
+                    for (int d = 0; d < DIM; ++d) acc[d] += fcp * g.dx[d];
+                }
+            }
+        }
+        // Monaghan artificial viscosity (artificialViscosity.f90:80-110)
+        if (P.arti_visc) {
+            double xv = g.dx[0] * dv[0] + g.dx[1] * dv[1] + (DIM == 3 ? g.dx[2] * dv[2] : 0.0);
+            if (xv < 0.0) {
+                double hij = 0.5 * (qi.w + qj.w);
+                double rhoij = 0.5 * (thi.x + thj.x);
+                double cij = 0.5 * (thi.z + thj.z);
+                double ph = P.av_psi * hij;
+                double phi = hij * xv / (g.r2 + ph * ph);
+                double PI_ij = (P.av_beta * phi - P.av_alpha * cij) * phi / rhoij;
+                for (int d = 0; d < DIM; ++d) acc[d] -= mj * PI_ij * g.gw[d];
+                de_av += 0.5 * mj * PI_ij * dvg;
+            }
+        }
+        // artificial stress (artificialTerm.f90:72-93)
+        if (as_on) {
+            const int tj = D.meta[j].x;
+            if (tj >= 100) {
+                double F = pow(g.w / wdelta, P.as_exponent);
+                const double* Rj = D.asR + (int64_t)j * 4;
+                for (int d = 0; d < 2; ++d) {
+                    double sv = 0.0;
+                    for (int e = 0; e < 2; ++e) sv += (Ri[d * 2 + e] + Rj[d * 2 + e]) * g.gw[e];
+                    acc[d] += mj * F * sv;
+                }
+            }
+        }
+    }
+    if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100))
+        acc[DIM - 1] += P.g;
+    if (mt.z == 1) acc[0] = acc[1] = acc[2] = 0.0;   // fixedBoundary, boundaryCondition.f90:32-45
+    double drho = P.summation_density ? 0.0 : thi.x * sdiv_v;
+    D.dv[s] = make_double4(acc[0], acc[1], acc[2], drho);
+    double tds = (P.viscosity || !fluid) ? D.tds[s] : 0.0;
+    D.de[s] = tds + 0.5 * de_in + de_av;
+    D.div[s] = make_double2(-sdiv_x, -sdiv_v);
+}
+
+// Fluid force pass for the dam-break physics set: cubic-spline kernel,
+// continuity density, SPH = 2 pressure force, Monaghan viscosity, gravity,
+// wall/penalty/coupling pair forces, divergences. R = double (FP64) or float
+// (mixed: the pair arithmetic in FP32, dx formed in FP64 then rounded).
+template <typename R> struct Vec4;
+template <> struct Vec4<double> { using T = double4; };
+template <> struct Vec4<float> { using T = float4; };
+
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
+    using R = typename std::conditional<MIXED, float, double>::type;
+    using R4 = typename Vec4<R>::T;
+    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    const int ti = mt.x;
+    const double4 qi = D.posh[s];
+    R4 vi, thi;
+    if (MIXED) { vi = *(const R4*)&D.vmf[s]; thi = *(const R4*)&D.thf[s]; }
+    else { vi = *(const R4*)&D.vm[s]; thi = *(const R4*)&D.th[s]; }
+    const R hi = (R)qi.w;
+    const R inv_h = (R)1 / hi;
+    // cubic spline (kernel.f90:30-71): grad W = G dx with
+    //   G = f/h^2 (-2 + 1.5 q)        q < 1
+    //   G = -f/(2h) (2 - q)^2 / r     1 <= q <= 2
+    const R fac = DIM == 2 ? (R)(15.0 / (7.0 * kPi)) * inv_h * inv_h
+                           : (R)(3.0 / (2.0 * kPi)) * inv_h * inv_h * inv_h;
+    const R g1 = fac * inv_h * inv_h;
+    const R g2 = (R)-0.5 * fac * inv_h;
+    const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, psi2 = (R)(P.av_psi * P.av_psi);
+    R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
+    const int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
+    const int* __restrict__ nl = D.nbr + s;
+    for (int k = 0; k < n; ++k) {
+        const int j = nl[(int64_t)k * P.ldn];
+        const double4 qj = D.posh[j];
+        R4 vj, thj;
+        if (MIXED) { vj = *(const R4*)&D.vmf[j]; thj = *(const R4*)&D.thf[j]; }
+        else { vj = *(const R4*)&D.vm[j]; thj = *(const R4*)&D.th[j]; }
+        const R dx = (R)(qi.x - qj.x), dy = (R)(qi.y - qj.y), dz = DIM == 3 ? (R)(qi.z - qj.z) : (R)0;
+        const R r2 = dx * dx + dy * dy + dz * dz;
+        const R r = fast_sqrt(r2);
+        const R q = r * inv_h;
+        const R t = (R)2 - q;
+        const R G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
+        const R gx = G * dx, gy = G * dy, gz = G * dz;
+        const R dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = DIM == 3 ? vi.z - vj.z : (R)0;
+        const R dvg = dvx * gx + dvy * gy + dvz * gz;
+        const R dxg = dx * gx + dy * gy + dz * gz;
+        const R mj = vj.w;
+        sv += thj.w * dvg;          // continuity / div v (density.f90:98-101)
+        sx += thj.w * dxg;          // div x (divergence.f90:17-27)
+        const R aux = thi.y + thj.y;  // p_i/rho_i^2 + p_j/rho_j^2 (internalForce.f90:236)
+        // Monaghan viscosity when approaching (artificialViscosity.f90:80-110)
+        const R xv = dx * dvx + dy * dvy + dz * dvz;
+        const R hij = (R)0.5 * (hi + (R)qj.w);
+        const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
+        const R Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
+                                     fast_rcp((R)0.5 * (thi.x + thj.x))
+                               : (R)0;
+        const R cf = mj * (aux + Pi);
+        ax -= cf * gx;
+        ay -= cf * gy;
+        az -= cf * gz;
+        de_in += mj * aux * dvg;
+        de_av += (R)0.5 * mj * Pi * dvg;
+        if (P.pair_ext) {
+            const int tj = D.meta[j].x;
+            R fe = 0;
+            if (ti > 0 && tj < 0 && (double)r < P.lj_r0) {
+                R rr = (R)P.lj_r0 / r;
+                fe += (R)P.lj_factor * ((R)pow((double)rr, P.lj_p1) - (R)pow((double)rr, P.lj_p2)) / r2;
+            }
+            if (ti != tj && ti > 0 && tj > 0) {
+                if (ti < 100 && tj < 100) {
+                    R pe = (hi + (R)qj.w) / ((R)2 * r);
+                    if (pe >= (R)1)
+                        fe += (R)P.pen_factor * ((R)pow((double)pe, P.pen_n1) - (R)pow((double)pe, P.pen_n2)) / r2;
+                } else if (!(ti > 100 && tj > 100)) {
+                    R et = r / (R)0.75 / ((hi + (R)qj.w) * (R)0.5);
+                    R chi = (R)1 - r / (R)P.couple_delta;
+                    if (chi < 0 || chi > 1) chi = 0;
+                    R fc = 0;
+                    if (et <= 0 || et >= 2) fc = 0;
+                    else if (et <= (R)(2.0 / 3)) fc = (R)(2.0 / 3);
+                    else if (et <= 1) fc = 2 * et - (R)1.5 * et * et;
+                    else if (et < 2) fc = (R)0.5 * ((2 - et) * (2 - et));
+                    fe += (R)0.01f * (thi.z * thi.z) * chi * fc / r2;
+                }
+            }
+            ax += fe * dx;
+            ay += fe * dy;
+            az += fe * dz;
+        }
+    }
+    if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100)) {
+        if (DIM == 2) ay += (R)P.g;
+        else az += (R)P.g;
+    }
+    if (mt.z == 1) ax = ay = az = 0;
+    const R drho = thi.x * sv;
+    const R de = (R)0.5 * de_in + de_av;
+    if (MIXED) {
+        D.dvf[s] = make_float4((float)ax, (float)ay, (float)az, (float)drho);
+        D.def[s] = (float)de;
+    } else {
+        D.dv[s] = make_double4(ax, ay, az, drho);
+        D.de[s] = de;
+    }
+    D.div[s] = make_double2(-(double)sx, -(double)sv);
+}
+
+// Leap-frog correct (timeIntegration.f90:178-252) and bounding-box partials
+// of the new positions for the next step's grid.
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_correct(Params P, Dev D, int first) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (s < P.n) {
+        int4 mt = D.meta[s];
+        double4 q = D.posh[s];
+        if (mt.w < P.n_real) {
+            const double dt = P.dt;
+            const double fac = first ? dt / 2 : dt;
+            double4 vp = D.vprev[s];
+            double dvx, dvy, dvz, dee;
+            bool finite_rho;
+            if (MIXED) {
+                float4 dv = D.dvf[s];
+                dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
+                float dr = D.drho_prev[s] + (float)fac * dv.w;
+                D.drho[s] = dr;
+                finite_rho = isfinite(dr);
+            } else {
+                double4 dv = D.dv[s];
+                dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
+                if (!P.summation_density) D.rho[s] = vp.w + fac * dv.w;
+                finite_rho = isfinite(D.rho[s]);
+            }
+            double e = D.eprev[s] + fac * dee;
+            if (e < 0) e = 0;
+            D.e[s] = e;
+            double4 v = D.vm[s];
+            v.x = vp.x + fac * dvx;
+            v.y = vp.y + fac * dvy;
+            if (DIM == 3) v.z = vp.z + fac * dvz;
+            D.vm[s] = v;
+            double4 ds = D.disp[s];
+            if (first) {
+                ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
+            } else {
+                ds.x = ds.x + dt * v.x; ds.y = ds.y + dt * v.y;
+                if (DIM == 3) ds.z = ds.z + dt * v.z;
+            }
+            D.disp[s] = ds;
+            q.x = q.x + dt * v.x;
+            q.y = q.y + dt * v.y;
+            if (DIM == 3) q.z = q.z + dt * v.z;
+            D.posh[s] = q;
+            if (!MIXED && P.has_solid && mt.x > 100) {
+                const Material* m = find_mat(P, mt.x);
+                if (m) {
+                    double* S = D.shear + (int64_t)s * DIM * DIM;
+                    const double* Sp = D.shear_prev + (int64_t)s * DIM * DIM;
+                    const double* dS = D.dsdt + (int64_t)s * DIM * DIM;
+                    for (int k = 0; k < DIM * DIM; ++k) S[k] = (first ? S[k] : Sp[k]) + fac * dS[k];
+                    if (m->sigma_y > 0) {
+                        double J2 = 0.0;
+                        for (int k = 0; k < DIM * DIM; ++k) J2 += S[k] * S[k];
+                        double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
+                        for (int k = 0; k < DIM * DIM; ++k) S[k] = S[k] * sc;
+                    }
+                }
+            }
+            if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
+                atomicOr(D.err, kErrNonFinite);
+        }
+        mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
+    }
+    bbox_block_reduce(mn, mx, D.partial, blockIdx.x);
+}
+
+// Neighbour statistics of the last search (print_statistics, nnps.f90:377-416):
+// out[0] total pairs, out[1] max, out[2] min, out[3] targets with none.
+__global__ __launch_bounds__(kBlock) void k_stats(Params P, Dev D, unsigned long long* out) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    int c = (mt.y == -1) ? 0 : D.ncnt[s];
+    atomicAdd(&out[0], (unsigned long long)c);
+    atomicMax(&out[1], (unsigned long long)c);
+    atomicMin(&out[2], (unsigned long long)c);
+    if (c == 0) atomicAdd(&out[3], 1ull);
+}
+
+// --------------------------------------------------------------------- host
+
+namespace {
+
+const char* kFieldNames[SPHGPU_F_NFIELDS] = {"x", "v", "rho", "p", "e", "mass", "h", "c",
+                                             "disp", "stress", "shear", "dvdt", "drhodt",
+                                             "dedt", "dsdt", "divx", "divv", "eta"};
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct sphgpu_ctx {
+    sphgpu_config cfg{};
+    Params P{};
+    int64_t capacity = 0;
+    int64_t n = 0, n_real = 0, n_dummy = 0;
+    int dim = 2;
+    hipStream_t stream = nullptr;
+    Dev D{};
+    std::vector<void*> allocs;
+    int64_t ncell_alloc = 0;
+    int64_t nbr_alloc = 0;
+    int nblk = 0;
+    int64_t step = 1;
+    bool has_state = false;
+    hipEvent_t ev[4]{};
+    double last_advance_ms = 0, last_force_ms = 0;
+    int64_t last_force_launches = 0;
+    bool time_force = false;
+    bool force_general = false;   // testing switch: use the general force kernel
+    std::string err;
+    int* h_err = nullptr;   // pinned
+};
+
+static int fail(sphgpu_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, SPHGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static int dalloc(sphgpu_ctx* c, T** p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIPCHK(c, hipMalloc(&q, bytes));
+    HIPCHK(c, hipMemsetAsync(q, 0, bytes, c->stream));
+    c->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+static void free_all(sphgpu_ctx* c) {
+    for (void* q : c->allocs) hipFree(q);
+    c->allocs.clear();
+    c->D = Dev{};
+    c->ncell_alloc = 0;
+    c->nbr_alloc = 0;
+}
+
+static int validate_config(const sphgpu_config* cfg, std::string* why) {
+    if (cfg->dim != 2 && cfg->dim != 3) { *why = "dim must be 2 or 3"; return SPHGPU_E_BAD_CONFIG; }
+    if (cfg->skf < 1 || cfg->skf > 3) { *why = "skf must be 1, 2 or 3"; return SPHGPU_E_BAD_CONFIG; }
+    if (cfg->pa_sph != 2) { *why = "only SPH = 2 (internalForce.f90:228-274) is supported"; return SPHGPU_E_BAD_CONFIG; }
+    if (cfg->artificial_stress && cfg->dim != 2) {
+        *why = "artificial stress is 2D only (artificialTerm.f90:94-95)";
+        return SPHGPU_E_BAD_CONFIG;
+    }
+    if (cfg->n_materials < 0 || cfg->n_materials > SPHGPU_MAX_MATERIALS) {
+        *why = "n_materials out of range";
+        return SPHGPU_E_BAD_CONFIG;
+    }
+    if (cfg->precision != SPHGPU_FP64 && cfg->precision != SPHGPU_MIXED) {
+        *why = "unknown precision";
+        return SPHGPU_E_BAD_CONFIG;
+    }
+    if (!(cfg->dt > 0)) { *why = "dt must be positive"; return SPHGPU_E_BAD_CONFIG; }
+    return 0;
+}
+
+extern "C" int sphgpu_abi_version(void) { return SPHGPU_ABI_VERSION; }
+
+extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_ctx** out) {
+    if (!cfg || !out || capacity <= 0) return SPHGPU_E_ARG;
+    std::string why;
+    int rc = validate_config(cfg, &why);
+    *out = nullptr;
+    sphgpu_ctx* c = new (std::nothrow) sphgpu_ctx();
+    if (!c) return SPHGPU_E_ARG;
+    if (rc) {
+        fprintf(stderr, "sphgpu_create: %s\n", why.c_str());
+        delete c;
+        return rc;
+    }
+    c->cfg = *cfg;
+    c->dim = cfg->dim;
+    c->capacity = capacity;
+    const char* fg = getenv("SPHGPU_FORCE_GENERAL");
+    c->force_general = fg && fg[0] == '1';
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SPHGPU_E_HIP;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return SPHGPU_E_HIP; }
+    if (hipHostMalloc((void**)&c->h_err, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return SPHGPU_E_HIP;
+    }
+    *out = c;
+    return 0;
+}
+
+static void fill_params(sphgpu_ctx* c) {
+    const sphgpu_config& g = c->cfg;
+    Params& P = c->P;
+    P.dim = g.dim;
+    P.skf = g.skf;
+    P.scale_k = (g.skf == 1) ? 2 : 3;   // asymmetricSearch.f90:178-185
+    P.n = (int)c->n;
+    P.n_real = (int)c->n_real;
+    P.nmat = g.n_materials;
+    P.summation_density = g.summation_density;
+    P.normalize_density = g.normalize_density;
+    P.viscosity = g.viscosity;
+    P.gravity = g.gravity;
+    P.gravity_mode = g.gravity_mode;
+    P.external_force = g.external_force;
+    P.arti_visc = g.artificial_viscosity;
+    P.arti_stress = g.artificial_stress;
+    P.kgc = g.kernel_correction;
+    P.g = g.gravity_g;
+    P.av_alpha = g.av_alpha; P.av_beta = g.av_beta; P.av_psi = g.av_psi;
+    P.lj_factor = g.lj_factor; P.lj_r0 = g.lj_r0; P.lj_p1 = g.lj_p1; P.lj_p2 = g.lj_p2;
+    P.pen_factor = g.pen_factor; P.pen_n1 = g.pen_n1; P.pen_n2 = g.pen_n2;
+    P.couple_delta = g.couple_delta;
+    P.as_lambda = g.as_lambda; P.as_delta = g.as_delta; P.as_exponent = g.as_exponent;
+    P.dt = g.dt;
+    P.mixed = g.precision == SPHGPU_MIXED;
+}
+
+// The fused fluid force kernel covers the dam-break physics set; anything
+// else (physical viscosity, solids, kernel correction, summation density,
+// artificial stress, Gauss/quintic kernels) takes the general kernels.
+static bool fluid_fast_path(const sphgpu_ctx* c) {
+    const sphgpu_config& g = c->cfg;
+    return g.skf == 1 && !g.viscosity && !c->P.has_solid && !g.kernel_correction &&
+           !g.summation_density && !g.artificial_stress && !c->force_general;
+}
+
+// Our search grid: cells of cutoff/kWin (slightly enlarged) over the upload
+// bounding box plus a margin; keys outside are clamped to the border cells,
+// which keeps every neighbour within the +-kWin sweep.
+static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
+    const int dim = c->dim;
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, hmax = 0;
+    for (int d = 0; d < dim; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
+    for (int64_t i = 0; i < c->n; ++i) {
+        for (int d = 0; d < dim; ++d) {
+            double v = x[i * dim + d];
+            if (v < mn[d]) mn[d] = v;
+            if (v > mx[d]) mx[d] = v;
+        }
+        hmax = std::max(hmax, h[i]);
+    }
+    double cs = hmax * c->P.scale_k / kWin * (1.0 + 1e-9);
+    if (!(cs > 0)) cs = 1.0;
+    double lo[3], ext[3];
+    for (int d = 0; d < dim; ++d) {
+        double span = mx[d] - mn[d];
+        double margin = 0.1 * span + 2 * cs;
+        lo[d] = mn[d] - margin;
+        ext[d] = span + 2 * margin;
+    }
+    const int64_t budget = std::max<int64_t>(8 * c->n, 4096);
+    for (;;) {
+        int64_t cells = 1;
+        for (int d = 0; d < dim; ++d) cells *= (int64_t)std::ceil(ext[d] / cs) + 1;
+        if (cells <= budget && cells < (1LL << 31) - 2) break;
+        cs *= 1.25;
+    }
+    Params& P = c->P;
+    for (int d = 0; d < 3; ++d) { P.org[d] = 0; P.nc[d] = 1; }
+    for (int d = 0; d < dim; ++d) {
+        P.org[d] = lo[d];
+        P.nc[d] = (int)std::ceil(ext[d] / cs) + 1;
+    }
+    P.inv_cs = 1.0 / cs;
+    P.ncell = (int64_t)P.nc[0] * P.nc[1] * P.nc[2];
+}
+
+template <int DIM>
+static void launch_sort(sphgpu_ctx* c) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
+    hipLaunchKernelGGL(k_keys<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    const int m = (int)P.ncell + 1;
+    const int sb = (m + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb);
+    hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
+                       c->D.start, c->D.fill, (int)P.n);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    hipLaunchKernelGGL(k_cellsort, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                       c->D.start, c->D.perm, m);
+}
+
+// One evaluation of the derivatives (mode bits as k_point_build).
+template <int DIM>
+static void launch_point_build(sphgpu_ctx* c, int mode) {
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    if (c->P.mixed)
+        hipLaunchKernelGGL((k_point_build<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, mode);
+    else
+        hipLaunchKernelGGL((k_point_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, mode);
+}
+
+template <int DIM>
+static void launch_derivs(sphgpu_ctx* c, int mode) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    launch_sort<DIM>(c);
+    launch_point_build<DIM>(c, mode);
+    const bool fast = fluid_fast_path(c);
+    if (!fast) {
+        if (P.kgc) hipLaunchKernelGGL(k_kgc<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        if (P.summation_density) {
+            hipLaunchKernelGGL(k_sumdens<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+            hipLaunchKernelGGL(k_apply_sumdens, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        }
+        if (P.viscosity || P.has_solid)
+            hipLaunchKernelGGL(k_strain<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        if (P.has_solid)
+            hipLaunchKernelGGL(k_solid_point<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    }
+    if (c->time_force) hipEventRecord(c->ev[2], c->stream);
+    if (P.mixed)
+        hipLaunchKernelGGL((k_force_fluid<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (fast)
+        hipLaunchKernelGGL((k_force_fluid<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else
+        hipLaunchKernelGGL(k_force<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (c->time_force) hipEventRecord(c->ev[3], c->stream);
+}
+
+template <int DIM>
+static void launch_step(sphgpu_ctx* c) {
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    int first = c->step == 1;
+    launch_derivs<DIM>(c, first ? 1 : 0);
+    if (c->P.mixed)
+        hipLaunchKernelGGL((k_correct<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
+    else
+        hipLaunchKernelGGL((k_correct<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
+}
+
+static int check_err(sphgpu_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->h_err, c->D.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int e = c->h_err[0];
+    if (e) {
+        HIPCHK(c, hipMemsetAsync(c->D.err, 0, sizeof(int), c->stream));
+        if (e & kErrNonFinite) return fail(c, SPHGPU_E_NONFINITE, "non-finite position, velocity or density");
+        if (e & kErrOutOfRange) return fail(c, SPHGPU_E_OUT_OF_RANGE, "particle out of range");
+        if (e & kErrPairCap) return fail(c, SPHGPU_E_PAIR_CAPACITY, "neighbour list capacity exceeded");
+    }
+    return 0;
+}
+
+static int alloc_state(sphgpu_ctx* c) {
+    free_all(c);
+    const int64_t N = c->capacity;
+    const int dim = c->dim;
+    const int64_t T = N * dim * dim;
+    Dev& D = c->D;
+    int rc = 0;
+    rc |= dalloc(c, &D.posh, N); rc |= dalloc(c, &D.vm, N); rc |= dalloc(c, &D.rho, N);
+    rc |= dalloc(c, &D.e, N); rc |= dalloc(c, &D.c, N); rc |= dalloc(c, &D.p, N);
+    rc |= dalloc(c, &D.meta, N); rc |= dalloc(c, &D.disp, N); rc |= dalloc(c, &D.dv, N);
+    rc |= dalloc(c, &D.de, N); rc |= dalloc(c, &D.vprev, N); rc |= dalloc(c, &D.eprev, N);
+    rc |= dalloc(c, &D.th, N); rc |= dalloc(c, &D.div, N); rc |= dalloc(c, &D.eta, N);
+    rc |= dalloc(c, &D.tds, N); rc |= dalloc(c, &D.key, N); rc |= dalloc(c, &D.refcell, N);
+    rc |= dalloc(c, &D.perm, N); rc |= dalloc(c, &D.ncnt, N);
+    bool need_tensor = c->cfg.viscosity || c->P.has_solid;
+    if (need_tensor) rc |= dalloc(c, &D.Ep, T);
+    // S, stress and dS/dt always exist (zero for fluids) so they can be read back
+    rc |= dalloc(c, &D.shear, T); rc |= dalloc(c, &D.stress, T);
+    rc |= dalloc(c, &D.dsdt, T);
+    if (c->P.has_solid) {
+        rc |= dalloc(c, &D.Ed, T); rc |= dalloc(c, &D.rot, T);
+        rc |= dalloc(c, &D.shear_prev, T); rc |= dalloc(c, &D.Sp, T);
+        rc |= dalloc(c, &D.asR, N * 4);
+    }
+    if (c->cfg.kernel_correction) rc |= dalloc(c, &D.kgcA, T);
+    if (c->P.mixed) {
+        rc |= dalloc(c, &D.vmf, N); rc |= dalloc(c, &D.thf, N); rc |= dalloc(c, &D.drho, N);
+        rc |= dalloc(c, &D.drho_prev, N); rc |= dalloc(c, &D.dvf, N); rc |= dalloc(c, &D.def, N);
+    }
+    if (c->cfg.summation_density) rc |= dalloc(c, &D.rho_new, N);
+    int nb = (int)((N + kBlock - 1) / kBlock);
+    rc |= dalloc(c, &D.partial, (size_t)nb * 6);
+    rc |= dalloc(c, &D.grid, 1);
+    rc |= dalloc(c, &D.err, 4);
+    rc |= dalloc(c, &D.maxcnt, 4);
+    rc |= dalloc(c, &D.mat, SPHGPU_MAX_MATERIALS);
+    if (rc) return SPHGPU_E_HIP;
+    Material tab[SPHGPU_MAX_MATERIALS];
+    for (int k = 0; k < SPHGPU_MAX_MATERIALS; ++k) {
+        const sphgpu_material& m = c->cfg.mat[k];
+        tab[k] = Material{m.type, m.eos, m.rho0, m.K, m.gamma, m.eta, m.sigma_y};
+    }
+    HIPCHK(c, hipMemcpyAsync(D.mat, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream));
+    c->P.mat = D.mat;
+    return 0;
+}
+
+static int alloc_cells(sphgpu_ctx* c) {
+    const int64_t m = c->P.ncell + 1;
+    if (m <= c->ncell_alloc) return 0;
+    Dev& D = c->D;
+    int rc = 0;
+    rc |= dalloc(c, &D.count, m + 1);
+    rc |= dalloc(c, &D.start, m + 1);
+    rc |= dalloc(c, &D.fill, m + 1);
+    rc |= dalloc(c, &D.bsum, (m + kScanTile - 1) / kScanTile + 1);
+    if (rc) return SPHGPU_E_HIP;
+    c->ncell_alloc = m;
+    return 0;
+}
+
+static int alloc_nbr(sphgpu_ctx* c, int cap) {
+    int64_t need = (int64_t)cap * c->capacity;
+    if (need <= c->nbr_alloc) return 0;
+    if (dalloc(c, &c->D.nbr, (size_t)need)) return SPHGPU_E_HIP;
+    c->nbr_alloc = need;
+    return 0;
+}
+
+template <int DIM>
+static int size_neighbour_list(sphgpu_ctx* c) {
+    // count-only pass on the uploaded state, then capacity with headroom
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    launch_sort<DIM>(c);
+    launch_point_build<DIM>(c, 2 | 4);
+    HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int e = check_err(c);
+    if (e) return e;
+    int mx = c->h_err[1];
+    int cap = mx + std::max(8, mx / 2);
+    c->P.nbr_cap = cap;
+    c->P.ldn = c->capacity;
+    return alloc_nbr(c, cap);
+}
+
+extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, const double* x,
+                             const double* v, const double* mass, const double* rho,
+                             const double* p, const double* e, const double* cs,
+                             const double* h, const double* shear, const int32_t* type,
+                             const int32_t* state, const int32_t* boundary) {
+    if (!c) return SPHGPU_E_ARG;
+    if (!x || !mass || !rho || !h || !type || n_real < 0 || n_dummy < 0)
+        return fail(c, SPHGPU_E_ARG, "x, mass, rho, h and type are required");
+    const int64_t n = n_real + n_dummy;
+    if (n > c->capacity) return fail(c, SPHGPU_E_CAPACITY, "more particles than the context capacity");
+    if (n == 0) return fail(c, SPHGPU_E_ARG, "no particles");
+    if (n >= (1LL << 31) - 1) return fail(c, SPHGPU_E_CAPACITY, "particle count exceeds int32 range");
+    const int dim = c->dim;
+    c->n = n;
+    c->n_real = n_real;
+    c->n_dummy = n_dummy;
+    fill_params(c);
+    c->P.has_solid = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (type[i] > 100 || type[i] < -100) c->P.has_solid = 1;
+    {
+        bool multi = false;
+        int first_pos = 0;
+        for (int64_t i = 0; i < n; ++i)
+            if (type[i] > 0) {
+                if (first_pos == 0) first_pos = type[i];
+                else if (type[i] != first_pos) multi = true;
+            }
+        c->P.pair_ext = c->cfg.external_force && (c->cfg.lj_r0 > 0 || multi);
+    }
+    if (c->P.mixed && !fluid_fast_path(c))
+        return fail(c, SPHGPU_E_BAD_CONFIG,
+                    "mixed precision covers the fluid set (cubic spline, continuity density, "
+                    "no physical viscosity, solids, kernel correction or artificial stress)");
+    if (c->cfg.kernel_correction)
+        for (int64_t i = 0; i < n_real; ++i)
+            if (type[i] <= 100)
+                return fail(c, SPHGPU_E_BAD_CONFIG,
+                            "kernel gradient correction with non-solid real particles "
+                            "(the reference divides by a zero matrix there)");
+    for (int64_t i = 0; i < n * dim; ++i)
+        if (!std::isfinite(x[i])) return fail(c, SPHGPU_E_NONFINITE, "non-finite position");
+    plan_grid(c, x, h);
+    int rc = alloc_state(c);
+    if (rc) return rc;
+    rc = alloc_cells(c);
+    if (rc) return rc;
+    // order the slots by our cell key once, so that the sweep reads nearby memory
+    std::vector<uint64_t> keyed(n);
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t cc[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            double t = std::floor((x[i * dim + d] - c->P.org[d]) * c->P.inv_cs);
+            t = std::min(std::max(t, 0.0), (double)(c->P.nc[d] - 1));
+            cc[d] = (int64_t)t;
+        }
+        uint64_t key = (uint64_t)(cc[0] + c->P.nc[0] * (cc[1] + (int64_t)c->P.nc[1] * cc[2]));
+        keyed[i] = (key << 32) | (uint64_t)i;
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<double4> posh(n), vm(n), disp(n), zero4(n);
+    std::vector<double> vrho(n), ve(n), vc(n), vp(n);
+    std::vector<int4> meta(n);
+    std::vector<double> vshear((size_t)n * dim * dim, 0.0);
+    for (int64_t s = 0; s < n; ++s) {
+        int64_t i = (int64_t)(keyed[s] & 0xffffffffu);
+        posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
+        double vx = v ? v[i * dim] : 0, vy = v ? v[i * dim + 1] : 0, vz = (v && dim == 3) ? v[i * dim + 2] : 0;
+        vm[s] = make_double4(vx, vy, vz, mass[i]);
+        disp[s] = make_double4(0, 0, 0, 0);
+        zero4[s] = make_double4(0, 0, 0, 0);
+        vrho[s] = rho[i];
+        ve[s] = e ? e[i] : 0.0;
+        vc[s] = cs ? cs[i] : 0.0;
+        vp[s] = p ? p[i] : 0.0;
+        meta[s] = make_int4(type[i], state ? state[i] : 0, boundary ? boundary[i] : 0, (int)i);
+        if (shear)
+            for (int q = 0; q < dim * dim; ++q) vshear[s * dim * dim + q] = shear[i * dim * dim + q];
+    }
+    Dev& D = c->D;
+    HIPCHK(c, hipMemcpyAsync(D.posh, posh.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.vm, vm.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.disp, disp.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.dv, zero4.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.rho, vrho.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.e, ve.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.c, vc.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.p, vp.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.meta, meta.data(), n * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(D.shear, vshear.data(), vshear.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->P.mixed) {
+        std::vector<float> dr(n);
+        for (int64_t s = 0; s < n; ++s) {
+            const sphgpu_material* m = nullptr;
+            int a = std::abs(meta[s].x);
+            for (int k = 0; k < c->cfg.n_materials; ++k)
+                if (c->cfg.mat[k].type == a) m = &c->cfg.mat[k];
+            double r0 = m ? m->rho0 : 0.0;   // the same reference density eos_mixed uses
+            dr[s] = (float)(vrho[s] - r0);
+        }
+        HIPCHK(c, hipMemcpyAsync(D.drho, dr.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(D.err, 0, sizeof(int) * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->step = 1;
+    c->has_state = true;
+    rc = dim == 2 ? size_neighbour_list<2>(c) : size_neighbour_list<3>(c);
+    if (rc) { c->has_state = false; return rc; }
+    // the grid partials of the uploaded positions seed the first step
+    const int nb = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    HIPCHK(c, hipGetLastError());
+    return check_err(c);
+}
+
+extern "C" int sphgpu_derivatives(sphgpu_ctx* c) {
+    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    if (c->dim == 2) launch_derivs<2>(c, 2);
+    else launch_derivs<3>(c, 2);
+    HIPCHK(c, hipGetLastError());
+    return check_err(c);
+}
+
+extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
+    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
+    if (nsteps < 0) return fail(c, SPHGPU_E_ARG, "negative step count");
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    // positions may have been changed by sphgpu_derivatives' caller: refresh partials
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    double force_ms = 0;
+    int64_t force_n = 0;
+    for (int k = 0; k < nsteps; ++k) {
+        c->time_force = (k == nsteps - 1);
+        if (c->dim == 2) launch_step<2>(c);
+        else launch_step<3>(c);
+        c->step++;
+    }
+    c->time_force = false;
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipGetLastError());
+    int rc = check_err(c);
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->last_advance_ms = ms;
+    if (nsteps > 0) {
+        float fm = 0;
+        hipEventElapsedTime(&fm, c->ev[2], c->ev[3]);
+        force_ms = fm;
+        force_n = 1;
+    }
+    c->last_force_ms = force_ms;
+    c->last_force_launches = force_n;
+    return rc;
+}
+
+extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
+    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
+    if (!dst || field < 0 || field >= SPHGPU_F_NFIELDS) return fail(c, SPHGPU_E_ARG, "bad field");
+    const int64_t n = c->n;
+    const int dim = c->dim;
+    std::vector<int4> meta(n);
+    HIPCHK(c, hipMemcpyAsync(meta.data(), c->D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    auto get4 = [&](double4* src, int which) -> int {  // which: 0 xyz, 1 w
+        std::vector<double4> h(n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < n; ++s) {
+            int64_t i = meta[s].w;
+            if (which == 0) {
+                dst[i * dim] = h[s].x;
+                dst[i * dim + 1] = h[s].y;
+                if (dim == 3) dst[i * dim + 2] = h[s].z;
+            } else {
+                dst[i] = h[s].w;
+            }
+        }
+        return 0;
+    };
+    auto get1 = [&](double* src) -> int {
+        std::vector<double> h(n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = h[s];
+        return 0;
+    };
+    auto getT = [&](double* src) -> int {
+        const int64_t nn = (int64_t)dim * dim;
+        if (!src) {
+            std::memset(dst, 0, sizeof(double) * n * nn);
+            return 0;
+        }
+        std::vector<double> h(n * nn);
+        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < n; ++s)
+            for (int q = 0; q < nn; ++q) dst[meta[s].w * nn + q] = h[s * nn + q];
+        return 0;
+    };
+    auto getf4 = [&](float4* src, int which) -> int {  // which: 0 xyz, 1 w
+        std::vector<float4> h(n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < n; ++s) {
+            int64_t i = meta[s].w;
+            if (which == 0) {
+                dst[i * dim] = h[s].x;
+                dst[i * dim + 1] = h[s].y;
+                if (dim == 3) dst[i * dim + 2] = h[s].z;
+            } else {
+                dst[i] = h[s].w;
+            }
+        }
+        return 0;
+    };
+    auto getf1 = [&](float* src, bool add_rho0) -> int {
+        std::vector<float> h(n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < n; ++s) {
+            double r0 = 0.0;
+            if (add_rho0) {
+                int a = std::abs(meta[s].x);
+                for (int k = 0; k < c->cfg.n_materials; ++k)
+                    if (c->cfg.mat[k].type == a) r0 = c->cfg.mat[k].rho0;
+            }
+            dst[meta[s].w] = r0 + (double)h[s];
+        }
+        return 0;
+    };
+    if (c->P.mixed) {
+        switch (field) {
+        case SPHGPU_F_DVDT: return getf4(c->D.dvf, 0);
+        case SPHGPU_F_DRHODT: return getf4(c->D.dvf, 1);
+        case SPHGPU_F_RHO: return getf1(c->D.drho, true);
+        case SPHGPU_F_DEDT: return getf1(c->D.def, false);
+        default: break;
+        }
+    }
+    auto get2 = [&](int which) -> int {
+        std::vector<double2> h(n);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->D.div, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = which ? h[s].y : h[s].x;
+        return 0;
+    };
+    switch (field) {
+    case SPHGPU_F_X: return get4(c->D.posh, 0);
+    case SPHGPU_F_H: return get4(c->D.posh, 1);
+    case SPHGPU_F_V: return get4(c->D.vm, 0);
+    case SPHGPU_F_MASS: return get4(c->D.vm, 1);
+    case SPHGPU_F_DISP: return get4(c->D.disp, 0);
+    case SPHGPU_F_DVDT: return get4(c->D.dv, 0);
+    case SPHGPU_F_DRHODT: return get4(c->D.dv, 1);
+    case SPHGPU_F_RHO: return get1(c->D.rho);
+    case SPHGPU_F_P: return get1(c->D.p);
+    case SPHGPU_F_E: return get1(c->D.e);
+    case SPHGPU_F_C: return get1(c->D.c);
+    case SPHGPU_F_DEDT: return get1(c->D.de);
+    case SPHGPU_F_ETA: return get1(c->D.eta);
+    case SPHGPU_F_STRESS: return getT(c->D.stress);
+    case SPHGPU_F_SHEAR: return getT(c->D.shear);
+    case SPHGPU_F_DSDT: return getT(c->D.dsdt);
+    case SPHGPU_F_DIVX: return get2(0);
+    case SPHGPU_F_DIVV: return get2(1);
+    default: break;
+    }
+    return fail(c, SPHGPU_E_ARG, std::string("field not available: ") + kFieldNames[field]);
+}
+
+extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr, int64_t cap,
+                                   int64_t* needed) {
+    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
+    if (!offsets) return fail(c, SPHGPU_E_ARG, "offsets required");
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    if (c->dim == 2) {
+        launch_sort<2>(c);
+        launch_point_build<2>(c, 2);
+    } else {
+        launch_sort<3>(c);
+        launch_point_build<3>(c, 2);
+    }
+    HIPCHK(c, hipGetLastError());
+    int rc = check_err(c);
+    if (rc) return rc;
+    const int64_t n = c->n, nr = c->n_real;
+    std::vector<int4> meta(n);
+    std::vector<int> cnt(n);
+    HIPCHK(c, hipMemcpyAsync(meta.data(), c->D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->D.ncnt, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int64_t> slot_of(nr, -1);
+    for (int64_t s = 0; s < n; ++s)
+        if (meta[s].w < nr) slot_of[meta[s].w] = s;
+    offsets[0] = 0;
+    for (int64_t i = 0; i < nr; ++i) {
+        int64_t s = slot_of[i];
+        int k = (meta[s].y == -1) ? 0 : cnt[s];
+        offsets[i + 1] = offsets[i] + k;
+    }
+    if (needed) *needed = offsets[nr];
+    if (!nbr || offsets[nr] > cap) return fail(c, SPHGPU_E_PAIR_CAPACITY, "pair buffer too small");
+    const int64_t ld = c->P.ldn, lcap = c->P.nbr_cap;
+    std::vector<int> all((size_t)(ld * lcap));
+    HIPCHK(c, hipMemcpyAsync(all.data(), c->D.nbr, all.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < nr; ++i) {
+        int64_t s = slot_of[i];
+        int64_t o = offsets[i];
+        int k = (int)(offsets[i + 1] - offsets[i]);
+        for (int q = 0; q < k; ++q) nbr[o + q] = meta[all[(size_t)(q * ld + s)]].w;
+        std::sort(nbr + o, nbr + o + k);
+    }
+    return 0;
+}
+
+extern "C" int sphgpu_neighbour_stats(sphgpu_ctx* c, int64_t* out4) {
+    if (!c || !c->has_state || !out4) return fail(c, SPHGPU_E_ARG, "bad argument");
+    unsigned long long* d = nullptr;
+    unsigned long long init[4] = {0, 0, ~0ull, 0};
+    HIPCHK(c, hipMalloc(&d, sizeof(init)));
+    HIPCHK(c, hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_stats, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, d);
+    unsigned long long h[4];
+    hipError_t e1 = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(c, SPHGPU_E_HIP, "neighbour statistics");
+    for (int k = 0; k < 4; ++k) out4[k] = (int64_t)h[k];
+    if (c->n_real == 0) out4[2] = 0;
+    return 0;
+}
+
+extern "C" int sphgpu_grid_info(sphgpu_ctx* c, double* out9) {
+    if (!c || !c->has_state || !out9) return fail(c, SPHGPU_E_ARG, "bad argument");
+    RefGrid g;
+    HIPCHK(c, hipMemcpyAsync(&g, c->D.grid, sizeof(RefGrid), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; ++d) {
+        out9[d] = g.minc[d];
+        out9[3 + d] = g.maxc[d];
+        out9[6 + d] = g.num[d];
+    }
+    return 0;
+}
+
+extern "C" int sphgpu_timing(sphgpu_ctx* c, double* advance_ms, double* force_kernel_ms,
+                             int64_t* force_launches) {
+    if (!c) return SPHGPU_E_ARG;
+    if (advance_ms) *advance_ms = c->last_advance_ms;
+    if (force_kernel_ms) *force_kernel_ms = c->last_force_ms;
+    if (force_launches) *force_launches = c->last_force_launches;
+    return 0;
+}
+
+extern "C" int64_t sphgpu_step(sphgpu_ctx* c) { return c ? c->step : -1; }
+
+extern "C" const char* sphgpu_last_error(sphgpu_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->h_err) hipHostFree(c->h_err);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}

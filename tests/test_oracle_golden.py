@@ -1,0 +1,152 @@
+"""The oracle against the golden fixtures written by the compiled reference
+(tests/golden/make_golden.py runs oracle/_ref/ref_driver, i.e. the reference's
+own Fortran modules). Full-array SHA-256 equality means bit-for-bit equality
+with the reference; the sampled values give a readable diff when it fails.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from mysph_amd import cases
+from tests.golden.make_golden import beam_velocity, custom_bggs_inputs
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sha(a) -> str:
+    a = np.ascontiguousarray(a)
+    a = a.astype("<f8") if a.dtype.kind == "f" else a.astype("<i4")
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def load(name):
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    with open(os.path.join(GOLD, name + ".json")) as f:
+        meta = json.load(f)
+    return z, meta
+
+
+def check_field(meta, z, key, full, sample):
+    """Bit-equality via the hash; on mismatch report the sampled difference."""
+    if sha(full) == meta[key.replace("@", "sha_")]:
+        return
+    ref = z[key.replace("@", "")]
+    got = full[::sample] if full.ndim else full
+    diff = np.max(np.abs(np.asarray(got).reshape(ref.shape) - ref) / (np.abs(ref).max() + 1e-300))
+    pytest.fail(f"{key}: not bit-identical to the reference (sampled rel diff {diff:.3e})")
+
+
+@pytest.fixture(scope="module")
+def dam():
+    return load("ref_dam_break")
+
+
+def test_reference_case_generator_is_bit_exact(dam):
+    _, meta = dam
+    c = cases.reference_dam_break()
+    assert c["n_real"] == 15376 and c["n_dummy"] == 5734
+    for k in ("x", "mass", "rho", "p", "h", "c", "type"):
+        assert sha(c[k].ravel()) == meta["init_sha_" + k], k
+
+
+def test_bggs_pairs_bit_exact(dam):
+    z, meta = dam
+    c = cases.reference_dam_break()
+    off, nbr, w, dw, _ = oracle.bggs(2, 1, c["n_real"], c["x"], c["h"], c["state"])
+    cnt = np.diff(off).astype(np.int32)
+    assert sha(cnt) == meta["pairs_sha_count"]
+    assert sha(nbr + 1) == meta["pairs_sha_nbr"]      # the reference is 1-based
+    assert sha(w) == meta["pairs_sha_w"]
+    assert sha(dw.ravel()) == meta["pairs_sha_dwdx"]
+    assert int(cnt.sum()) == meta["pairs_total"] == 122267
+
+
+def test_step1_derivatives_bit_exact(dam):
+    z, meta = dam
+    c = cases.reference_dam_break()
+    nt = c["n_real"]
+    s = oracle.OracleSim(c)
+    s.derivatives()
+    for k in ("dvdt", "drhodt", "dedt", "p", "divx", "divv", "eta"):
+        a = s[k][:nt]
+        assert sha(a.ravel()) == meta["deriv_sha_" + k], k
+    assert sha(s["p"]) == meta["deriv_sha_p_all"]
+
+
+@pytest.mark.parametrize("nsteps", [1, 10, 100, 1000])
+def test_leap_frog_steps_bit_exact(dam, nsteps):
+    z, meta = dam
+    c = cases.reference_dam_break()
+    nt = c["n_real"]
+    oracle.set_threads(os.cpu_count() or 1)
+    s = oracle.OracleSim(c)
+    s.run(nsteps)
+    for k in ("x", "v", "rho", "p", "e", "disp", "divx", "divv", "stress"):
+        a = s[k][:nt].reshape(nt, -1)
+        check_field(meta, z, f"step{nsteps}_@{k}", a, meta["sample"])
+
+
+def test_summation_density_with_shepard_bit_exact():
+    z, meta = load("ref_dam_break_sumdens")
+    c = cases.reference_dam_break()
+    cfg = c["config"].copy()
+    cfg.summation_density = 1
+    cfg.normalize_density = 1
+    nt = c["n_real"]
+    for n in (1, 10):
+        s = oracle.OracleSim(c, cfg)
+        s.run(n)
+        for k in ("x", "v", "rho", "p", "e"):
+            check_field(meta, z, f"step{n}_@{k}", s[k][:nt].reshape(nt, -1), meta["sample"])
+
+
+@pytest.mark.parametrize("tag", ["ref_can_beam", "ref_can_beam_as", "ref_can_beam_3d"])
+def test_solid_beams_bit_exact(tag):
+    z, meta = load(tag)
+    c = cases.reference_can_beam_3d() if tag.endswith("3d") else cases.reference_can_beam()
+    c["v"] = beam_velocity(c["x"])
+    cfg = c["config"].copy()
+    cfg.artificial_stress = 1 if tag.endswith("_as") else 0
+    nt = c["n_real"]
+    s = oracle.OracleSim(c, cfg)
+    s.derivatives()
+    for k in ("dvdt", "drhodt", "dedt", "dsdt"):
+        a = s[k][:nt].reshape(nt, -1)
+        assert sha(a.ravel()) == meta["deriv_sha_" + k], k
+    for n in (1, 20, 200):
+        s = oracle.OracleSim(c, cfg)
+        s.run(n)
+        for k in ("x", "v", "rho", "p", "e", "stress"):
+            check_field(meta, z, f"step{n}_@{k}", s[k][:nt].reshape(nt, -1), meta["sample"])
+
+
+def test_custom_bggs_sets_bit_exact():
+    z, meta = load("ref_bggs_custom")
+    sets = custom_bggs_inputs()
+    assert len(sets) == meta["nsets"]
+    for k, s in enumerate(sets):
+        assert sha(s["x"]) == meta[f"set{k}_sha_x"]
+        off, nbr, w, dw, _ = oracle.bggs(s["dim"], s["skf"], s["ntotal"], s["x"], s["h"],
+                                         s["state"])
+        assert np.array_equal(np.diff(off), z[f"set{k}_count"]), k
+        assert sha(nbr + 1) == meta[f"set{k}_sha_nbr"], k
+        assert sha(w) == meta[f"set{k}_sha_w"], k
+        assert sha(dw.ravel()) == meta[f"set{k}_sha_dwdx"], k
+
+
+def test_custom_bggs_window_misses_neighbours():
+    """Set 0 was chosen so that int(h/cell)+1 cells do not cover 2h: the
+    reference misses some pairs with r < 2h. The oracle reproduces that."""
+    s = custom_bggs_inputs()[0]
+    off, nbr, _, _, grid = oracle.bggs(2, 1, s["ntotal"], s["x"], s["h"], s["state"])
+    x = s["x"]
+    nt = s["ntotal"]
+    d = np.sqrt(((x[:nt, None, :] - x[None, :, :]) ** 2).sum(-1))
+    brute = (d < 2 * s["h"][:nt, None]).sum() - nt
+    assert len(nbr) < brute

@@ -1,0 +1,270 @@
+"""Checks of the oracle that do not depend on the reference: closed forms,
+conservation laws and exact integrator behaviour. Each is chosen so that a
+plausible slip (a dropped factor, a sign, a swapped index) fails it.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from mysph_amd import cases
+from mysph_amd import config as C
+
+
+# ------------------------------------------------------------------ kernels
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("skf,support", [(1, 2.0), (2, 3.0), (3, 3.0)])
+def test_kernel_normalisation(dim, skf, support):
+    """int W dV = 1 over the support (midpoint rule on a fine lattice)."""
+    h = 1.0
+    n = 120 if dim == 2 else 48
+    g = (np.arange(n) + 0.5) / n * 2 * support - support
+    cell = (2 * support / n) ** dim
+    grids = np.meshgrid(*([g] * dim), indexing="ij")
+    pts = np.stack([x.ravel() for x in grids], -1)
+    r = np.sqrt((pts ** 2).sum(-1))
+    tot = 0.0
+    for ri, p in zip(r, pts):
+        if ri < support * h:
+            w, _ = oracle.kernel(dim, skf, ri, p, h)
+            tot += w
+    tol = 2e-3 if skf == 1 else 1e-2   # Gauss is truncated at 3h
+    assert abs(tot * cell - 1.0) < tol
+
+
+@pytest.mark.parametrize("skf", [1, 3])
+def test_kernel_gradient_is_dw_dr_times_dx_over_r(skf):
+    h = 0.7
+    for r in (0.1, 0.5, 0.69, 0.71, 1.0, 1.3, 1.9):
+        if skf == 1 and r >= 2 * h:
+            continue
+        if skf == 3 and r < h:
+            continue   # the inner quintic branch is the reference's own polynomial, below
+        dx = np.array([r * 0.6, r * 0.8])
+        w, g = oracle.kernel(2, skf, r, dx, h)
+        eps = 1e-6
+        wp, _ = oracle.kernel(2, skf, r + eps, dx, h)
+        wm, _ = oracle.kernel(2, skf, r - eps, dx, h)
+        dwdr = (wp - wm) / (2 * eps)
+        assert np.allclose(g, dwdr * dx / r, rtol=1e-6, atol=1e-9)
+
+
+def test_quintic_inner_gradient_is_the_reference_polynomial():
+    """kernel.f90:119-120 uses (-120 + 120 q - 50 q^2)/h^2 dx for q < 1, while
+    dW/dr dx/r would give (-120 + 120 q^2 - 50 q^3)/h^2 dx. The restatement
+    keeps the reference's polynomial; this pins it (and would catch a silent
+    'fix')."""
+    h = 0.7
+    f = 7.0 / (478.0 * math.pi * h * h)
+    for r in (0.05, 0.3, 0.6):
+        q = r / h
+        dx = np.array([r * 0.6, r * 0.8])
+        _, g = oracle.kernel(2, 3, r, dx, h)
+        assert np.allclose(g, f * (-120 + 120 * q - 50 * q * q) / h ** 2 * dx, rtol=1e-14)
+        true = f * (-120 + 120 * q ** 2 - 50 * q ** 3) / h ** 2 * dx
+        assert not np.allclose(g, true, rtol=1e-6)
+
+
+def test_cubic_kernel_closed_form_points():
+    h = 1.0
+    f2 = 15.0 / (7.0 * math.pi)
+    w0, _ = oracle.kernel(2, 1, 0.0, np.zeros(2), h)
+    assert w0 == pytest.approx(f2 * 2 / 3, rel=1e-15)
+    w1, _ = oracle.kernel(2, 1, 1.0, np.array([1.0, 0.0]), h)
+    assert w1 == pytest.approx(f2 / 6, rel=1e-15)
+    w2, g2 = oracle.kernel(2, 1, 2.0, np.array([2.0, 0.0]), h)
+    assert w2 == 0.0 and np.all(g2 == 0.0)
+
+
+# --------------------------------------------------------------- EOS, case
+
+def test_tait_inverse_round_trip_and_zero_at_rest():
+    B = C.tait_B(50.0)
+    for p in (0.0, 1e3, 2.4e5, 1e6):
+        rho = math.pow(p / B + 1.0, 1.0 / 7.0) * 1000.0
+        p2 = B * (math.pow(rho / 1000.0, 7.0) - 1)
+        assert p2 == pytest.approx(p, rel=1e-12, abs=1e-9)
+
+
+# ----------------------------------------------------- conservation laws
+
+def blob(n=400, seed=3, dim=2):
+    """Free fluid particles (no walls, no gravity) with random velocities."""
+    rng = np.random.default_rng(seed)
+    side = int(round(n ** (1 / dim)))
+    g = np.stack(np.meshgrid(*[np.arange(side)] * dim, indexing="ij"), -1).reshape(-1, dim)
+    dx = 0.1
+    x = (g + 0.5) * dx + rng.uniform(-0.1, 0.1, g.shape) * dx
+    m = len(x)
+    case = {
+        "dim": dim, "n_real": m, "n_dummy": 0, "x": x,
+        "v": rng.normal(0, 1.0, x.shape), "mass": np.full(m, 1000.0 * dx ** dim),
+        "rho": 1000.0 + rng.normal(0, 5.0, m), "p": np.zeros(m), "e": np.full(m, 1.0),
+        "c": np.full(m, 20.0), "h": np.full(m, 1.2 * dx),
+        "type": np.full(m, 2, np.int32), "state": np.zeros(m, np.int32),
+        "boundary": np.zeros(m, np.int32),
+    }
+    cfg = C.base_config(dim)
+    cfg.summation_density = 0
+    cfg.artificial_viscosity = 1
+    cfg.av_alpha, cfg.av_beta = 1.0, 1.0
+    cfg.add_material(2, C.EOS_TAIT, 1000.0, C.tait_B(20.0), 7.0)
+    case["config"] = cfg
+    return case
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_momentum_and_energy_conserved(dim):
+    """With equal h, the pressure and artificial-viscosity pair terms are
+    antisymmetric: sum m_i a_i = 0 and d/dt (KE + sum m e) = 0."""
+    case = blob(dim=dim, n=400 if dim == 2 else 512)
+    s = oracle.OracleSim(case)
+    s.derivatives()
+    m = case["mass"]
+    a = s["dvdt"]
+    ref = np.abs(m[:, None] * a).sum()
+    assert np.abs((m[:, None] * a).sum(0)).max() < 1e-12 * ref
+    dke = (m[:, None] * case["v"] * a).sum()
+    die = (m * s["dedt"]).sum()
+    assert abs(dke + die) < 1e-11 * (abs(dke) + abs(die))
+
+
+def test_uniform_velocity_gives_no_density_change():
+    case = blob()
+    case["v"][:] = np.array([0.3, -1.2])
+    s = oracle.OracleSim(case)
+    s.derivatives()
+    assert np.abs(s["drhodt"]).max() < 1e-9
+
+
+def test_av_acts_only_when_approaching():
+    """Two particles: receding -> no viscous force; approaching -> the
+    Monaghan form with phi = h xv / (r^2 + (psi h)^2)."""
+    for sign in (+1, -1):
+        case = blob(n=4)
+        case["x"] = np.array([[0.0, 0.0], [0.15, 0.0], [5.0, 5.0], [5.15, 5.0]])
+        case["v"] = np.array([[0.0, 0.0], [sign * 1.0, 0.0], [0, 0], [0, 0]])
+        case["rho"][:] = 1000.0
+        cfg = case["config"].copy()
+        cfg.artificial_viscosity = 1
+        s = oracle.OracleSim(case, cfg)
+        s.derivatives()
+        a_av = s["dvdt"][0].copy()
+        cfg0 = cfg.copy()
+        cfg0.artificial_viscosity = 0
+        s0 = oracle.OracleSim(case, cfg0)
+        s0.derivatives()
+        d = a_av - s0["dvdt"][0]
+        if sign > 0:
+            assert np.all(d == 0)
+        else:
+            h, xv, r2 = 0.12, 0.15 * 1.0, 0.15 ** 2   # dx = x0-x1 = -0.15, dv = v0-v1 = +1
+            xv = -0.15 * 1.0
+            phi = h * xv / (r2 + (cfg.av_psi * h) ** 2)
+            PI = (cfg.av_beta * phi - cfg.av_alpha * 20.0) * phi / 1000.0
+            _, g = oracle.kernel(2, 1, 0.15, np.array([-0.15, 0.0]), h)
+            assert np.allclose(d, -case["mass"][1] * PI * g, rtol=1e-12)
+
+
+def test_wall_repulsion_single_pair():
+    """externalForce.f90:109-124: f = factor ((r0/r)^12 - (r0/r)^4) dx / r^2."""
+    case = blob(n=4)
+    case["x"] = np.array([[0.0, 0.0], [9.0, 9.0], [0.1, 0.05], [7.0, 7.0]])
+    case["type"] = np.array([2, 2, -2, -2], np.int32)
+    case["n_real"], case["n_dummy"] = 2, 2
+    case["v"][:] = 0
+    cfg = case["config"].copy()
+    cfg.artificial_viscosity = 0
+    cfg.external_force = 1
+    cfg.lj_factor, cfg.lj_r0 = 10.0, 0.2
+    s1 = oracle.OracleSim(case, cfg)
+    s1.derivatives()
+    cfg0 = cfg.copy()
+    cfg0.lj_r0 = 0.0
+    s0 = oracle.OracleSim(case, cfg0)
+    s0.derivatives()
+    dx = np.array([-0.1, -0.05])
+    r = np.sqrt((dx ** 2).sum())
+    expect = 10.0 * ((0.2 / r) ** 12 - (0.2 / r) ** 4) * dx / r ** 2
+    assert np.allclose(s1["dvdt"][0] - s0["dvdt"][0], expect, rtol=1e-12)
+
+
+def test_fluid_solid_coupling_single_pair():
+    """externalForce.f90:142-170 for one fluid target and one solid neighbour."""
+    case = blob(n=4)
+    case["x"] = np.array([[0.0, 0.0], [0.1, 0.0], [5, 5], [6, 6]])
+    case["type"] = np.array([2, 102, 2, 2], np.int32)
+    case["v"][:] = 0
+    cfg = case["config"].copy()
+    cfg.artificial_viscosity = 0
+    cfg.external_force = 1
+    cfg.couple_delta = 0.25
+    cfg.add_material(102, C.EOS_LINEAR, 1000.0, 1e4, 1.0, eta=1e5)
+    s = oracle.OracleSim(case, cfg)
+    s.derivatives()
+    cfg0 = cfg.copy()
+    cfg0.external_force = 0
+    s0 = oracle.OracleSim(case, cfg0)
+    s0.derivatives()
+    r, h = 0.1, 0.12
+    eta = r / 0.75 / h
+    fc = 2 * eta - 1.5 * eta * eta if eta <= 1 else 0.5 * (2 - eta) ** 2
+    chi = 1 - r / 0.25
+    expect = float(np.float32(0.01)) * 20.0 ** 2 * chi * fc * np.array([-0.1, 0.0]) / r ** 2
+    assert np.allclose(s["dvdt"][0] - s0["dvdt"][0], expect, rtol=1e-12)
+
+
+# --------------------------------------------------------------- integrator
+
+def test_leap_frog_exact_for_constant_acceleration():
+    """A lone particle under gravity: x_n = x0 + v0 t + g t^2 / 2 exactly
+    (up to rounding) with the reference's first-step / predict-correct scheme."""
+    case = blob(n=4)
+    case["x"] = np.array([[0.0, 0.0], [10.0, 3.0], [20.0, 7.0], [30.0, 1.0]])
+    case["v"] = np.array([[1.0, 2.0]] * 4)
+    cfg = case["config"].copy()
+    cfg.gravity = 1
+    cfg.external_force = 1
+    cfg.artificial_viscosity = 0
+    cfg.dt = 1e-3
+    s = oracle.OracleSim(case, cfg)
+    s.run(100)
+    t = 100 * cfg.dt
+    expect = np.array([0.0 + 1.0 * t, 0.0 + 2.0 * t - 0.5 * 9.81 * t * t])
+    assert np.allclose(s["x"][0], expect, rtol=0, atol=1e-12)
+    assert np.allclose(s["v"][0], [1.0, 2.0 - 9.81 * (t - cfg.dt / 2)], atol=1e-12)
+
+
+def test_thread_count_does_not_change_bits():
+    case = cases.dam_break_2d(dx=0.05)
+    out = []
+    for th in (1, 4):
+        oracle.set_threads(th)
+        s = oracle.OracleSim(case)
+        s.run(5)
+        out.append(np.concatenate([s[k].ravel() for k in ("x", "v", "rho")]))
+    oracle.set_threads(0)
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+
+
+def test_bggs_equals_brute_force_when_window_covers_cutoff():
+    rng = np.random.default_rng(7)
+    x = rng.random((300, 2))
+    h = np.full(300, 0.02)
+    off, nbr, _, _, _ = oracle.bggs(2, 1, 300, x, h)
+    d = np.sqrt(((x[:, None] - x[None]) ** 2).sum(-1))
+    brute = sorted((i, j) for i in range(300) for j in range(300) if i != j and d[i, j] < 0.04)
+    got = sorted((i, int(j)) for i in range(300) for j in nbr[off[i]:off[i + 1]])
+    assert got == brute
+
+
+def test_flat_bounding_box_is_an_error():
+    """Sources on one line make the reference's cell counts divide by zero
+    (asymmetricSearch.f90:103-105); the oracle reports it instead."""
+    x = np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]])
+    with pytest.raises(RuntimeError):
+        oracle.bggs(2, 1, 3, x, np.full(3, 0.5))
