@@ -29,6 +29,25 @@ METRIC = "particle-steps/s @1M 3D dam break, mixed vs FP64; p50/p99 batched-run 
 PEAK_FP64 = 78.6     # TFLOP/s, MI355X vector FP64
 PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32
 FLOP_PER_PAIR_3D = 81  # algorithmic FLOP per interacting pair (3D dam-break physics set)
+PEAK_HBM = 8.0e12    # B/s, MI355X HBM3E
+# HBM bytes per force-kernel dispatch, from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+# scaled by the same counters on a plain 1 GiB copy (counters are not readable
+# from inside a process, so the figure is recorded beside the kernel trace).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "force_kernel_hbm_traffic.json")
+
+
+def measured_traffic(mixed):
+    """HBM bytes per launch of the 3D fluid force kernel, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    tag = "k_force_fused<3, %s, false>" % ("true" if mixed else "false")
+    for name, rec in table.items():
+        if tag in name and rec.get("fetch_bytes") and rec.get("write_bytes") is not None:
+            return rec["fetch_bytes"] + rec["write_bytes"]
+    return None
 
 
 def log(*a):
@@ -56,6 +75,26 @@ def run_gpu(case, precision, warmup, steps):
     sim.close()
     return dict(wall_s=t1 - t0, event_ms=adv_ms, force_ms=force_ms, pairs=stats["pairs"],
                 max_nbr=stats["max"], rho=rho, label=C.FP64 if precision == 0 else C.MIXED)
+
+
+def accuracy(case, nsteps):
+    """Mixed against FP64 on the GPU after nsteps from the same state (the FP64
+    GPU path is checked against the oracle by the parity tests)."""
+    from mysph_amd import config as C
+    from mysph_amd.sphgpu import Simulation
+    nf = case["n_real"]
+    out = {}
+    for name, prec in (("fp64", C.FP64), ("mixed", C.MIXED)):
+        cfg = case["config"].copy()
+        cfg.precision = prec
+        with Simulation(case, cfg) as sim:
+            sim.advance(nsteps)
+            out[name] = {k: sim.get(k)[:nf] for k in ("rho", "x", "v")}
+    rho_err = np.max(np.abs(out["mixed"]["rho"] - out["fp64"]["rho"]) / out["fp64"]["rho"])
+    x_err = np.max(np.abs(out["mixed"]["x"] - out["fp64"]["x"])) / np.max(np.abs(out["fp64"]["x"]))
+    v_err = np.max(np.abs(out["mixed"]["v"] - out["fp64"]["v"])) / np.max(np.abs(out["fp64"]["v"]))
+    return {"steps": nsteps, "mixed_vs_fp64_rho_max_rel": float(rho_err),
+            "mixed_vs_fp64_x_rel": float(x_err), "mixed_vs_fp64_v_rel": float(v_err)}
 
 
 def cpu_baseline(case, seconds_target=20.0):
@@ -88,7 +127,20 @@ def main():
     ap.add_argument("--dx", type=float, default=0.0075)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="untimed run of both precisions to report the mixed density error")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design note, not part of this build)",
@@ -113,6 +165,7 @@ def main():
     peak = PEAK_FP32 if head is res.get("mixed") else PEAK_FP64
     flops = FLOP_PER_PAIR_3D * head["pairs"]
     achieved = flops / (head["force_ms"] * 1e-3) / 1e12 if head["force_ms"] > 0 else 0.0
+    traffic = measured_traffic(head is res.get("mixed"))
     out = {
         "metric": METRIC,
         "value": head["pss"],
@@ -136,16 +189,20 @@ def main():
             "dt": float(case["config"].dt),
         },
         "roofline": {
-            "kernel": "k_force_fluid (fused continuity + pressure + Monaghan viscosity pass)",
+            "kernel": "k_force_fused (exact neighbour filter + continuity + pressure + Monaghan viscosity)",
             "bound": "valu",
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": traffic,
             "flop_per_pair": FLOP_PER_PAIR_3D,
             "pairs_per_launch": head["pairs"],
             "launch_ms": head["force_ms"],
+            "hbm_gbs_measured": (traffic / (head["force_ms"] * 1e-3) / 1e9
+                                 if traffic and head["force_ms"] > 0 else None),
+            "hbm_frac_measured": (traffic / (head["force_ms"] * 1e-3) / PEAK_HBM
+                                  if traffic and head["force_ms"] > 0 else None),
         },
     }
     if "fp64" in res:
@@ -156,6 +213,8 @@ def main():
         out["mixed_over_fp64"] = res["mixed"]["pss"] / res["fp64"]["pss"]
         d = np.max(np.abs(res["mixed"]["rho"] - res["fp64"]["rho"]) / res["fp64"]["rho"])
         out["mixed_rho_rel_err_vs_fp64"] = float(d)
+    if "fp64" in res and "mixed" in res and args.accuracy_steps > 0:
+        out["accuracy"] = accuracy(case, args.accuracy_steps)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(case)

@@ -1,0 +1,429 @@
+// Fluid fast path of libsphgpu (included by sphgpu.hip after the shared
+// helpers): a candidate list within cutoff + skin, rebuilt only when some
+// particle has moved skin/2 since the last build, and one fused pass that
+// applies the reference's exact neighbour rule to the candidates and
+// accumulates the pair physics of the accepted ones.
+//
+// Exactness: a pair with r < cut at step n had r < cut + 2 max|x(n) - x(build)|
+// < cut + skin at the build, and the build keeps every candidate with r below
+// cut + skin (FP32 test with a bounding margin), so the candidate list holds
+// every reference neighbour; the fused pass then accepts exactly those with
+// the reference's arithmetic (asymmetricSearch.f90:227-237) and window
+// (:209-217).
+
+// Predict + EOS + pair inputs for every particle, the reference cell (for the
+// window check), and the largest displacement since the last list build.
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_point_fast(Params P, Dev D, int mode) {
+    // mode bit 0: first step (no predict); bit 1: no predict at all
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    bool moved = false;
+    if (s < P.n) {
+        const int4 mt = D.meta[s];
+        const bool real = mt.w < P.n_real;
+        if (MIXED) {
+            if (real && !(mode & 2)) predict_mixed<DIM>(P, D, s, mode & 1);
+            eos_mixed(P, D, s, mt.x);
+        } else {
+            if (real && !(mode & 2)) predict_one<DIM>(P, D, s, mt, mode & 1);
+            eos_th(P, D, s, mt);
+        }
+        const double4 q = D.posh[s];
+        if (mt.y != -1 && D.grid->need_win) {
+            const RefGrid g = *D.grid;
+            bool bad = false;
+            const double xs[3] = {q.x, q.y, q.z};
+            int rc[3] = {1, 1, 1};
+            for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
+            if (bad) atomicOr(D.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+                                                                                      : kErrNonFinite);
+            D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
+        }
+        const double4 b = D.xb[s];
+        const double ex = q.x - b.x, ey = q.y - b.y, ez = q.z - b.z;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        moved = !(d2 < P.disp_thr2);   // NaN also forces a rebuild
+    }
+    // one atomic per wave, and only when some lane crossed the threshold
+    if (__any(moved) && (threadIdx.x & 63) == 0 && __atomic_load_n(&D.vflag[1], __ATOMIC_RELAXED) == 0ull)
+        atomicOr(&D.vflag[1], 1ull);
+}
+
+// Candidate list: every j within cut_i + skin of i (superset is harmless), from
+// the +-win sweep over the cell-ordered FP32 positions. Also records the
+// build positions of all particles.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_verlet_build(Params P, Dev D, Gate G, int count_only) {
+    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n || !gate_open(G)) return;
+    const double4 qi = D.posh[s];
+    D.xb[s] = qi;
+    const int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    if (mt.y == -1) {
+        D.vcnt[s] = 0;
+        return;
+    }
+    const double cutv = qi.w * (double)P.scale_k + P.skin;
+    const RefGrid g = *D.grid;
+    const double u = 5.9604644775390625e-8;
+    const double e = g.pos_err + u * cutv;
+    const double M = 2.0 * 1.7320508075688772 * cutv * e + 3.0 * e * e + 3.0 * u * cutv * cutv;
+    const float lim = (float)(cutv * cutv + 2.0 * M + 1e-6 * cutv * cutv);
+    const float xf = (float)(qi.x - P.org[0]), yf = (float)(qi.y - P.org[1]), zf = (float)(qi.z - P.org[2]);
+    const uint32_t key = D.key[s];
+    const int cx = (int)(key % (uint32_t)P.nc[0]);
+    const int rest = (int)(key / (uint32_t)P.nc[0]);
+    const int cy = rest % P.nc[1];
+    const int cz = rest / P.nc[1];
+    const int W = P.win;
+    const int zlo = DIM == 3 ? max(cz - W, 0) : 0, zhi = DIM == 3 ? min(cz + W, P.nc[2] - 1) : 0;
+    const int ylo = max(cy - W, 0), yhi = min(cy + W, P.nc[1] - 1);
+    const int xlo = max(cx - W, 0), xhi = min(cx + W, P.nc[0] - 1);
+    const int cap = count_only ? 0 : P.verlet_cap;
+    int cnt = 0;
+    auto test = [&](const float4 cj) {
+        const float dx = xf - cj.x, dy = yf - cj.y, dz = DIM == 3 ? zf - cj.z : 0.0f;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const int j = __float_as_int(cj.w);
+        if (r2 < lim && j != s) {
+            if (cnt < cap) D.vnbr[(int64_t)cnt * P.ldn + s] = j;
+            ++cnt;
+        }
+    };
+    int z = zlo, y = ylo;
+    int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
+    int kb = D.start[row + xlo], ke = D.start[row + xhi + 1];
+    for (;;) {
+        int ny = y + 1, nz = z;
+        if (ny > yhi) { ny = ylo; ++nz; }
+        const bool more = nz <= zhi;
+        int nkb = 0, nke = 0;
+        if (more) {
+            const int64_t nrow = (int64_t)P.nc[0] * ((int64_t)ny + (int64_t)P.nc[1] * nz);
+            nkb = D.start[nrow + xlo];
+            nke = D.start[nrow + xhi + 1];
+        }
+        int k = kb;
+        for (; k + 1 < ke; k += 2) {
+            const float4 c0 = D.cpos[k];
+            const float4 c1 = D.cpos[k + 1];
+            test(c0);
+            test(c1);
+        }
+        if (k < ke) test(D.cpos[k]);
+        if (!more) break;
+        y = ny; z = nz; kb = nkb; ke = nke;
+    }
+    D.vcnt[s] = cnt;
+    if (!count_only && cnt > P.verlet_cap) atomicOr(D.err, kErrPairCap);
+    atomicMax(D.maxcnt, cnt);
+}
+
+// Exact neighbour test of one candidate in FP64, the reference's arithmetic:
+// squared distance summed in axis order, correctly rounded, no contraction;
+// accept sqrt(dr) < cut. Returns dr (or -1 if rejected).
+template <int DIM>
+__device__ __forceinline__ double exact_dr(const double4& qi, const double4& qj, double cut,
+                                           double lo2, double hi2) {
+    const double dx0 = __dsub_rn(qi.x, qj.x);
+    const double dx1 = __dsub_rn(qi.y, qj.y);
+    double dr = __dadd_rn(__dmul_rn(dx0, dx0), __dmul_rn(dx1, dx1));
+    if (DIM == 3) {
+        const double dx2 = __dsub_rn(qi.z, qj.z);
+        dr = __dadd_rn(dr, __dmul_rn(dx2, dx2));
+    }
+    if (dr < lo2) return dr;
+    if (!(dr < hi2)) return -1.0;
+    return (__dsqrt_rn(dr) < cut) ? dr : -1.0;
+}
+
+// Fused pass over the candidate list: exact neighbour test + window, then the
+// same physics as k_force_fluid for the accepted pairs.
+template <int DIM, bool MIXED, bool PAIR_EXT>
+__global__ __launch_bounds__(kBlock) void k_force_fused(Params P, Dev D) {
+    using R = typename std::conditional<MIXED, float, double>::type;
+    using R4 = typename Vec4<R>::T;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // every gated kernel of this step has run: clear the gate for the next
+        D.vflag[0] = 0ull;
+        D.vflag[1] = 0ull;
+    }
+    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    const int ti = mt.x;
+    const double4 qi = D.posh[s];
+    const double cut = qi.w * (double)P.scale_k;
+    const double cut2 = cut * cut;
+    const double lo2 = cut2 * (1.0 - 1e-12), hi2 = cut2 * (1.0 + 1e-12);
+    R4 vi, thi;
+    R pix = 0, piy = 0, piz = 0;
+    int cxi = 0, cyi = 0, czi = 0;
+    bool esc_i = false;
+    float lof = 0.0f, hif = 0.0f;
+    if (MIXED) {
+        vi = *(const R4*)&D.vmf[s];
+        thi = *(const R4*)&D.thf[s];
+        const float4 pf = D.posf[s];
+        pix = pf.x; piy = pf.y; piz = pf.z;
+        const uint32_t pk = __float_as_uint(pf.w);
+        esc_i = pk >> 31;
+        unpack_cell<DIM>(pk, cxi, cyi, czi);
+        // FP32 r^2 from cell offsets is within M of the FP64 one
+        const double u = 5.9604644775390625e-8;
+        const double e = 2.0 * u * P.cs + u * cut;
+        const double M = 2.0 * 1.7320508075688772 * cut * e + 3.0 * e * e + 3.0 * u * cut2;
+        lof = (float)(cut2 - 2.0 * M - 1e-6 * cut2);
+        hif = (float)(cut2 + 2.0 * M + 1e-6 * cut2);
+    } else {
+        vi = *(const R4*)&D.vm[s];
+        thi = *(const R4*)&D.th[s];
+    }
+    const R hi = (R)qi.w;
+    // reference window (only when it can exclude a pair of this particle)
+    const RefGrid g = *D.grid;
+    int wr[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
+    bool need_win = false;
+    const uint32_t rci = g.need_win ? D.refcell[s] : 0u;
+    for (int d = 0; d < DIM && g.need_win; ++d) {
+        const double span = g.maxc[d] - g.minc[d];
+        wr[d] = (int)(qi.w / span * g.num[d]) + 1;
+        rc[d] = ref_axis(rci, d);
+        const double reach = cut / (span / g.num[d]) * (1.0 + 1e-12) + 1e-9;
+        if (wr[d] < (int)ceil(reach)) need_win = true;
+    }
+    const R csf = (R)P.cs;
+    const R inv_h = (R)1 / hi;
+    const R fac = DIM == 2 ? (R)(15.0 / (7.0 * kPi)) * inv_h * inv_h
+                           : (R)(3.0 / (2.0 * kPi)) * inv_h * inv_h * inv_h;
+    const R g1 = fac * inv_h * inv_h;
+    const R g2 = (R)-0.5 * fac * inv_h;
+    const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, psi2 = (R)(P.av_psi * P.av_psi);
+    R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
+    int acc_n = 0;
+    const int n = (mt.y == -1) ? 0 : min(D.vcnt[s], P.verlet_cap);
+    const int* __restrict__ nl = D.vnbr + s;
+    int jn = n > 0 ? nl[0] : 0;
+    for (int k = 0; k < n; ++k) {
+        const int j = jn;
+        if (k + 1 < n) jn = nl[(int64_t)(k + 1) * P.ldn];
+        R dx, dy, dz, r2;
+        if (MIXED) {
+            const float4 pf = D.posf[j];
+            const uint32_t pk = __float_as_uint(pf.w);
+            int cxj, cyj, czj;
+            unpack_cell<DIM>(pk, cxj, cyj, czj);
+            dx = (pix - pf.x) + (R)(cxi - cxj) * csf;
+            dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
+            dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
+            r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < hif) && !esc_i && !(pk >> 31)) continue;
+            if (!(r2 < lof) || esc_i || (pk >> 31)) {
+                if (exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) continue;
+            }
+        } else {
+            const double4 qj = D.posh[j];
+            const double dr = exact_dr<DIM>(qi, qj, cut, lo2, hi2);
+            if (dr < 0.0) continue;
+            dx = (R)(qi.x - qj.x);
+            dy = (R)(qi.y - qj.y);
+            dz = DIM == 3 ? (R)(qi.z - qj.z) : (R)0;
+            r2 = dr;
+        }
+        if (need_win) {
+            const uint32_t rcj = D.refcell[j];
+            bool inwin = true;
+            for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
+            if (!inwin) continue;
+        }
+        ++acc_n;
+        R4 vj, thj;
+        R hj, Vj;
+        if (MIXED) {
+            vj = *(const R4*)&D.vmf[j];
+            thj = *(const R4*)&D.thf[j];
+            hj = thj.w;
+            Vj = vj.w * fast_rcp(thj.x);
+        } else {
+            vj = *(const R4*)&D.vm[j];
+            thj = *(const R4*)&D.th[j];
+            hj = (R)D.posh[j].w;
+            Vj = thj.w;
+        }
+        const R r = fast_sqrt(r2);
+        const R q = r * inv_h;
+        const R t = (R)2 - q;
+        const R G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
+        const R gx = G * dx, gy = G * dy, gz = G * dz;
+        const R dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = DIM == 3 ? vi.z - vj.z : (R)0;
+        const R dvg = dvx * gx + dvy * gy + dvz * gz;
+        const R dxg = dx * gx + dy * gy + dz * gz;
+        const R mj = vj.w;
+        sv += Vj * dvg;
+        sx += Vj * dxg;
+        const R aux = thi.y + thj.y;
+        const R xv = dx * dvx + dy * dvy + dz * dvz;
+        const R hij = (R)0.5 * (hi + hj);
+        const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
+        const R Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
+                                     fast_rcp((R)0.5 * (thi.x + thj.x))
+                               : (R)0;
+        const R cf = mj * (aux + Pi);
+        ax -= cf * gx;
+        ay -= cf * gy;
+        az -= cf * gz;
+        de_in += mj * aux * dvg;
+        de_av += (R)0.5 * mj * Pi * dvg;
+        if (PAIR_EXT) {
+            const int tj = D.meta[j].x;
+            R fe = 0;
+            if (ti > 0 && tj < 0 && (double)r < P.lj_r0) {
+                R rr = (R)P.lj_r0 / r;
+                fe += (R)P.lj_factor * ((R)pow((double)rr, P.lj_p1) - (R)pow((double)rr, P.lj_p2)) / r2;
+            }
+            if (ti != tj && ti > 0 && tj > 0) {
+                if (ti < 100 && tj < 100) {
+                    R pe = (hi + hj) / ((R)2 * r);
+                    if (pe >= (R)1)
+                        fe += (R)P.pen_factor * ((R)pow((double)pe, P.pen_n1) - (R)pow((double)pe, P.pen_n2)) / r2;
+                } else if (!(ti > 100 && tj > 100)) {
+                    R et = r / (R)0.75 / ((hi + hj) * (R)0.5);
+                    R chi = (R)1 - r / (R)P.couple_delta;
+                    if (chi < 0 || chi > 1) chi = 0;
+                    R fc = 0;
+                    if (et <= 0 || et >= 2) fc = 0;
+                    else if (et <= (R)(2.0 / 3)) fc = (R)(2.0 / 3);
+                    else if (et <= 1) fc = 2 * et - (R)1.5 * et * et;
+                    else if (et < 2) fc = (R)0.5 * ((2 - et) * (2 - et));
+                    fe += (R)0.01f * (thi.z * thi.z) * chi * fc / r2;
+                }
+            }
+            ax += fe * dx;
+            ay += fe * dy;
+            az += fe * dz;
+        }
+    }
+    if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100)) {
+        if (DIM == 2) ay += (R)P.g;
+        else az += (R)P.g;
+    }
+    if (mt.z == 1) ax = ay = az = 0;
+    const R drho = thi.x * sv;
+    const R de = (R)0.5 * de_in + de_av;
+    if (MIXED) {
+        D.dvf[s] = make_float4((float)ax, (float)ay, (float)az, (float)drho);
+        D.def[s] = (float)de;
+    } else {
+        D.dv[s] = make_double4(ax, ay, az, drho);
+        D.de[s] = de;
+    }
+    D.div[s] = make_double2(-(double)sx, -(double)sv);
+    D.ecnt[s] = acc_n;
+}
+
+// Leap-frog correct of step n-1 (timeIntegration.f90:178-252) fused with the
+// predict + EOS of step n (:137-170, internalForce.f90:103-134) and the
+// displacement check, for real particles; bounding-box partials of the new
+// positions for the grid of step n. Dummies are static: their pair inputs
+// were written by k_point_fast and do not change.
+template <int DIM, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_correct_predict(Params P, Dev D, int first) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool moved = false;
+    if (s < P.n) {
+        const int4 mt = D.meta[s];
+        double4 q = D.posh[s];
+        if (mt.w < P.n_real) {
+            const double dt = P.dt, h2 = dt / 2;
+            const double fac = first ? h2 : dt;
+            const double4 vp = D.vprev[s];
+            double dvx, dvy, dvz, dee;
+            bool finite_rho;
+            double4 v = D.vm[s];
+            double e, rho_c = 0.0;
+            if (MIXED) {
+                const float4 dv = D.dvf[s];
+                dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
+                float dr = D.drho_prev[s] + (float)fac * dv.w;   // correct
+                finite_rho = isfinite(dr);
+                D.drho_prev[s] = dr;
+                dr = dr + (float)h2 * dv.w;                      // predict of the next step
+                D.drho[s] = dr;
+            } else {
+                const double4 dv = D.dv[s];
+                dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
+                rho_c = vp.w + fac * dv.w;                      // correct
+                finite_rho = isfinite(rho_c);
+                D.rho[s] = rho_c + h2 * dv.w;                   // predict of the next step
+            }
+            e = D.eprev[s] + fac * dee;
+            if (e < 0) e = 0;
+            D.eprev[s] = e;
+            e = e + h2 * dee;
+            if (e < 0) e = 0;
+            D.e[s] = e;
+            v.x = vp.x + fac * dvx;
+            v.y = vp.y + fac * dvy;
+            if (DIM == 3) v.z = vp.z + fac * dvz;
+            double4 ds = D.disp[s];
+            if (first) {
+                ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
+            } else {
+                ds.x = ds.x + dt * v.x; ds.y = ds.y + dt * v.y;
+                if (DIM == 3) ds.z = ds.z + dt * v.z;
+            }
+            D.disp[s] = ds;
+            q.x = q.x + dt * v.x;
+            q.y = q.y + dt * v.y;
+            if (DIM == 3) q.z = q.z + dt * v.z;
+            D.posh[s] = q;
+            // predict of the next step: prev = corrected state, then half kick
+            D.vprev[s] = make_double4(v.x, v.y, v.z, rho_c);
+            v.x = v.x + h2 * dvx;
+            v.y = v.y + h2 * dvy;
+            if (DIM == 3) v.z = v.z + h2 * dvz;
+            D.vm[s] = v;
+            if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
+                atomicOr(D.err, kErrNonFinite);
+            if (MIXED) eos_mixed(P, D, s, mt.x);
+            else eos_th(P, D, s, mt);
+            const double4 b = D.xb[s];
+            const double ex = q.x - b.x, ey = q.y - b.y, ez = q.z - b.z;
+            moved = !(ex * ex + ey * ey + ez * ez < P.disp_thr2);
+        }
+        mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
+    }
+    if (__any(moved) && (threadIdx.x & 63) == 0 && __atomic_load_n(&D.vflag[1], __ATOMIC_RELAXED) == 0ull)
+        atomicOr(&D.vflag[1], 1ull);
+    bbox_block_reduce(mn, mx, D.partial, blockIdx.x);
+}
+
+// Reference cells of all particles, needed only when the reference window can
+// exclude pairs (grid flag); otherwise the launch returns at once.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_refcell(Params P, Dev D) {
+    if (!D.grid->need_win) return;
+    const RefGrid g = *D.grid;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+        const int4 mt = D.meta[s];
+        if (mt.y == -1) continue;
+        const double4 q = D.posh[s];
+        bool bad = false;
+        const double xs[3] = {q.x, q.y, q.z};
+        int rc[3] = {1, 1, 1};
+        for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
+        if (bad) atomicOr(D.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+                                                                                  : kErrNonFinite);
+        D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
+    }
+}
+
+__global__ void k_set_flag(unsigned long long* flag) {  // force the next rebuild
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        flag[0] = 1ull;
+        flag[1] = 0ull;
+    }
+}

@@ -13,7 +13,7 @@ constexpr double kPi = 3.141592653589793;                  // acos(-1.0_8), tool
 constexpr double kGmrM1 = 0.2000000476837158203125;        // real(1.2_4,8) - 1, asymmetricSearch.f90:21
 constexpr int kNpg = 3;                                      // asymmetricSearch.f90:22
 constexpr int kBlock = 256;
-constexpr int kWin = 2;   // our cells are cutoff/kWin wide: a +-kWin cell sweep covers the cutoff
+constexpr int kWinDefault = 3;   // cells are cutoff/win wide: a +-win cell sweep covers the cutoff
 constexpr int kNumXcd = 8;
 
 // error word bits (one word per context, OR-ed by kernels)
@@ -44,17 +44,37 @@ struct Params {
     // our search grid (fixed per upload; keys are clamped into it)
     double org[3];
     double inv_cs;
+    double cs;          // cell size (exactly representable in FP32)
     int nc[3];
+    int win;            // sweep half-width in cells
     int64_t ncell;      // nc[0]*nc[1]*nc[2]; cell index ncell holds State == -1 particles
     int nbr_cap;        // neighbour-list slots per particle
     int64_t ldn;        // leading dimension of the neighbour list (particle capacity)
+    // fluid fast path: candidate (Verlet) list within cutoff + skin, rebuilt
+    // when some particle has moved skin/2 since the last build
+    double skin;
+    double disp_thr2;   // (skin/2)^2, slightly reduced
+    int verlet_cap;
+    double hmax;        // largest smoothing length
 };
+
+// Rebuild gate of the candidate list: flag[0] forces a rebuild, flag[1] is set
+// when some particle has moved skin/2 since the last build. A null flag means
+// "always".
+struct Gate {
+    const unsigned long long* flag;
+};
+
+__device__ __forceinline__ bool gate_open(const Gate& g) {
+    return !g.flag || g.flag[0] != 0ull || g.flag[1] != 0ull;
+}
 
 // Reference background grid of one step (asymmetricSearch.f90:49-131).
 struct RefGrid {
     double minc[3], maxc[3];
     int num[3];
-    int pad;
+    int need_win;     // the reference window can exclude a pair (else it need not be checked)
+    double pos_err;   // bound on |fl32(x - org) - (x - org)| over all particles
 };
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {

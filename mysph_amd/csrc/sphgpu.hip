@@ -70,7 +70,6 @@ struct Dev {
     int* perm;
     int* count;
     int* start;
-    int* fill;
     int* bsum;
     int* nbr;
     int* ncnt;
@@ -81,12 +80,20 @@ struct Dev {
     Material* mat;
     // mixed precision: FP32 copies read by the pair kernels, and FP32 density
     // deviation / derivatives (positions and velocities stay FP64)
+    float4* cpos;     // FP32 positions relative to the grid origin, in cell order; w = slot
+    float4* posf;     // mixed: offset from own cell corner (xyz) + packed cell (w)
     float4* vmf;      // vx, vy, vz, mass
-    float4* thf;      // rho, p/rho^2, c, m/rho
+    float4* thf;      // rho, p/rho^2, c, h
     float* drho;      // rho - rho0
     float* drho_prev;
     float4* dvf;      // dvdt xyz, d(rho)/dt w
     float* def;       // de/dt
+    // fluid fast path
+    double4* xb;      // positions at the last candidate-list build
+    int* vnbr;        // candidate list, column-major like nbr
+    int* vcnt;
+    int* ecnt;        // accepted neighbours per particle in the last force pass
+    unsigned long long* vflag;   // rebuild gate (see Gate)
 };
 
 // Block min/max of positions -> partial[blockIdx*6 + (0..2 min, 3..5 max)].
@@ -183,16 +190,27 @@ __global__ __launch_bounds__(kBlock) void k_grid(Params P, Dev D, int nparts) {
                 g.num[d] = 1;
                 g.maxc[d] = g.minc[d] + 1.0;
             }
-        g.pad = 0;
+        // window w = int(h a) + 1 cells, reach of the cutoff = scale_k h a cells
+        // (a = num/span); it can drop a pair only if scale_k h a > int(h a) + 1,
+        // which needs scale_k h a >= 1 for some h <= hmax
+        g.need_win = 0;
+        for (int d = 0; d < DIM; ++d) {
+            double a = g.num[d] / (g.maxc[d] - g.minc[d]);
+            if (P.scale_k * P.hmax * a * (1.0 + 1e-9) >= 1.0) g.need_win = 1;
+        }
+        double X = 0.0;
+        for (int d = 0; d < DIM; ++d)
+            X = fmax(X, fmax(fabs(smx[d][0] - P.org[d]), fabs(smn[d][0] - P.org[d])));
+        g.pos_err = 2.0 * 5.9604644775390625e-8 * X;   // 2^-24 |x - org|, doubled
         *D.grid = g;
     }
 }
 
 // Our cell key (clamped into the fixed grid) and the reference cell.
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= P.n) return;
+__global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D, Gate G, int with_ref) {
+    if (!gate_open(G)) return;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
     double4 q = D.posh[s];
     int4 mt = D.meta[s];
     uint32_t key;
@@ -207,6 +225,9 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D) {
             c[d] = (int)t;
         }
         key = (uint32_t)((int64_t)c[0] + (int64_t)P.nc[0] * ((int64_t)c[1] + (int64_t)P.nc[1] * c[2]));
+    }
+    if (with_ref && mt.y != -1) {
+        double xs[3] = {q.x, q.y, q.z};
         const RefGrid g = *D.grid;
         bool bad = false;
         int rc[3] = {1, 1, 1};
@@ -217,7 +238,12 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D) {
     }
     D.key[s] = key;
     atomicAdd(&D.count[key], 1);
+    }
 }
+
+// Grid of the grid-stride kernels (gated kernels exit at once when no rebuild
+// is due, so a small grid keeps that cheap).
+constexpr int kStrideBlocks = 2048;
 
 // Exclusive scan of count[0..m) into start[0..m], 2048 entries per block.
 constexpr int kScanItems = 8;
@@ -243,17 +269,21 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
     return off + x - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan1(const int* __restrict__ count, int m, int* bsum) {
-    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+__global__ __launch_bounds__(kBlock) void k_scan1(const int* __restrict__ count, int m, int* bsum, Gate G) {
+    if (!gate_open(G)) return;
+    const int base = blockIdx.x * kScanTile;
     int s = 0;
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < m) s += count[base + k];
+    for (int k = 0; k < kScanItems; ++k) {
+        int i = base + threadIdx.x + k * kBlock;   // coalesced
+        if (i < m) s += count[i];
+    }
     int tot;
     block_exclusive_scan(s, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb) {
+__global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb, Gate G) {
+    if (!gate_open(G)) return;
     int carry = 0;
     for (int b0 = 0; b0 < nb; b0 += kBlock) {
         int b = b0 + threadIdx.x;
@@ -266,41 +296,55 @@ __global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan3(int* count, int m, const int* bsum, int* start,
-                                                  int* fill, int total) {
-    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+// Cell start table: start[c] = exclusive prefix of count; the tile goes through
+// LDS so that global loads and stores stay coalesced.
+__global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count, int m,
+                                                  const int* bsum, int* start, int total, Gate G) {
+    if (!gate_open(G)) return;
+    __shared__ int tile[kScanTile];
+    const int base = blockIdx.x * kScanTile;
+    for (int k = 0; k < kScanItems; ++k) {
+        int i = threadIdx.x + k * kBlock;
+        tile[i] = (base + i < m) ? count[base + i] : 0;
+    }
+    __syncthreads();
     int v[kScanItems];
     int s = 0;
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < m) ? count[base + k] : 0;
+        v[k] = tile[threadIdx.x * kScanItems + k];
         s += v[k];
     }
     int tot;
     int ex = block_exclusive_scan(s, &tot) + bsum[blockIdx.x];
     for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < m) {
-            start[base + k] = ex;
-            fill[base + k] = ex;
-            count[base + k] = 0;
-        }
+        tile[threadIdx.x * kScanItems + k] = ex;
         ex += v[k];
+    }
+    __syncthreads();
+    for (int k = 0; k < kScanItems; ++k) {
+        int i = threadIdx.x + k * kBlock;
+        if (base + i < m) start[base + i] = tile[i];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) start[m] = total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= P.n) return;
-    int pos = atomicAdd(&D.fill[D.key[s]], 1);
-    D.perm[pos] = s;
+// Place each particle in its cell. The decrement leaves every count at zero
+// again for the next step.
+__global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D, Gate G) {
+    if (!gate_open(G)) return;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+        uint32_t key = D.key[s];
+        int pos = D.start[key] + atomicSub(&D.count[key], 1) - 1;
+        D.perm[pos] = s;
+    }
 }
 
 // Ascending slot order inside each cell: the pair order, and so every sum,
 // is the same from run to run.
 __global__ __launch_bounds__(kBlock) void k_cellsort(const int* __restrict__ start, int* perm,
-                                                     int m) {
-    int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= m) return;
+                                                     int m, Gate G) {
+    if (!gate_open(G)) return;
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
     int a = start[c], b = start[c + 1];
     for (int i = a + 1; i < b; ++i) {
         int v = perm[i];
@@ -310,6 +354,19 @@ __global__ __launch_bounds__(kBlock) void k_cellsort(const int* __restrict__ sta
             --j;
         }
         perm[j + 1] = v;
+    }
+    }
+}
+
+// FP32 positions (relative to the grid origin) in cell order, with the slot
+// in w: the candidate loop of the search streams these 16-byte records.
+__global__ __launch_bounds__(kBlock) void k_cellpos(Params P, Dev D, Gate G) {
+    if (!gate_open(G)) return;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < P.n; k += gridDim.x * kBlock) {
+        int j = D.perm[k];
+        double4 q = D.posh[j];
+        D.cpos[k] = make_float4((float)(q.x - P.org[0]), (float)(q.y - P.org[1]),
+                                (float)(q.z - P.org[2]), __int_as_float(j));
     }
 }
 
@@ -413,12 +470,33 @@ __device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, 
     double4 v = D.vm[s];
     float mass = (float)v.w;
     float irho = 1.0f / rho;
-    D.thf[s] = make_float4(rho, p * irho * irho, (float)D.c[s], mass * irho);
+    double4 q = D.posh[s];
+    D.thf[s] = make_float4(rho, p * irho * irho, (float)D.c[s], (float)q.w);
     D.vmf[s] = make_float4((float)v.x, (float)v.y, (float)v.z, mass);
+    // offset from the particle's own cell corner, exact cell size cs: dx of a
+    // pair is (off_i - off_j) + (c_i - c_j) cs, accurate to ~1e-7 of a cell
+    double xs[3] = {q.x, q.y, q.z};
+    int c[3] = {0, 0, 0};
+    float off[3] = {0.0f, 0.0f, 0.0f};
+    bool escaped = false;
+    for (int d = 0; d < 3; ++d) {
+        if (d < P.dim) {
+            double t0 = floor((xs[d] - P.org[d]) * P.inv_cs);
+            double t = fmin(fmax(t0, 0.0), (double)(P.nc[d] - 1));
+            escaped |= t != t0;
+            c[d] = (int)t;
+            off[d] = (float)(xs[d] - P.org[d] - t * P.cs);
+        }
+    }
+    // bit 31: the particle is outside the grid (offset not bounded by a cell)
+    uint32_t pk = P.dim == 3 ? ((uint32_t)c[0] | ((uint32_t)c[1] << 11) | ((uint32_t)c[2] << 22))
+                             : ((uint32_t)c[0] | ((uint32_t)c[1] << 16));
+    if (escaped) pk |= 0x80000000u;
+    D.posf[s] = make_float4(off[0], off[1], off[2], __uint_as_float(pk));
 }
 
 // Predict + EOS for every particle, then the neighbour search for real
-// particles: candidates from the +-kWin cell sweep of our grid, accepted with
+// particles: candidates from the +-win cell sweep of our grid, accepted with
 // the reference rule r < scale_k * h_i (r from the sum of squares in axis order,
 // correctly rounded, no contraction), j != i, and j inside i's reference window
 // (asymmetricSearch.f90:209-237).
@@ -441,52 +519,97 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
         if (real) D.ncnt[s] = 0;
         return;
     }
-    double4 qi = D.posh[s];
-    double hi = qi.w;
-    double cut = hi * (double)P.scale_k;
-    double cut2 = cut * cut * (1.0 + 1e-12);
-    uint32_t key = D.key[s];
-    int cx = (int)(key % (uint32_t)P.nc[0]);
-    int rest = (int)(key / (uint32_t)P.nc[0]);
-    int cy = rest % P.nc[1];
-    int cz = rest / P.nc[1];
+    const double4 qi = D.posh[s];
+    const double hi = qi.w;
+    const double cut = hi * (double)P.scale_k;
+    const uint32_t key = D.key[s];
+    const int cx = (int)(key % (uint32_t)P.nc[0]);
+    const int rest = (int)(key / (uint32_t)P.nc[0]);
+    const int cy = rest % P.nc[1];
+    const int cz = rest / P.nc[1];
     const RefGrid g = *D.grid;
-    uint32_t rci = D.refcell[s];
+    const uint32_t rci = D.refcell[s];
     int wr[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
+    bool need_win = false;
     for (int d = 0; d < DIM; ++d) {
         // cellNumPerHSML, asymmetricSearch.f90:209-211
-        wr[d] = (int)(hi / (g.maxc[d] - g.minc[d]) * g.num[d]) + 1;
+        const double span = g.maxc[d] - g.minc[d];
+        wr[d] = (int)(hi / span * g.num[d]) + 1;
         rc[d] = ref_axis(rci, d);
+        // the window can only drop a pair if it is narrower than the cutoff
+        // measured in reference cells (rounded up, with slack for rounding)
+        const double reach = cut / (span / g.num[d]) * (1.0 + 1e-12) + 1e-9;
+        if (wr[d] < (int)ceil(reach)) need_win = true;
     }
+    // FP32 pre-test: accept if r2 < lo, reject if r2 >= hi, decide the band
+    // between in FP64 with the reference's arithmetic. M bounds |r2_f32 - r2|.
+    const double u = 5.9604644775390625e-8;
+    const double e = g.pos_err + u * cut;
+    const double M = 2.0 * 1.7320508075688772 * cut * e + 3.0 * e * e + 3.0 * u * cut * cut;
+    const double cut2 = cut * cut;
+    const double margin = 2.0 * M + 1e-6 * cut2;
+    const float lo = (float)(cut2 - margin), hi2 = (float)(cut2 + margin);
+    const float xf = (float)(qi.x - P.org[0]), yf = (float)(qi.y - P.org[1]),
+                zf = (float)(qi.z - P.org[2]);
     int cnt = 0;
     const int cap = (mode & 4) ? 0 : P.nbr_cap;
-    const int zlo = DIM == 3 ? max(cz - kWin, 0) : 0, zhi = DIM == 3 ? min(cz + kWin, P.nc[2] - 1) : 0;
-    const int ylo = max(cy - kWin, 0), yhi = min(cy + kWin, P.nc[1] - 1);
-    const int xlo = max(cx - kWin, 0), xhi = min(cx + kWin, P.nc[0] - 1);
-    for (int z = zlo; z <= zhi; ++z)
-        for (int y = ylo; y <= yhi; ++y) {
-            int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
-            int kb = D.start[row + xlo], ke = D.start[row + xhi + 1];
-            for (int k = kb; k < ke; ++k) {
-                int j = D.perm[k];
-                double4 qj = D.posh[j];
-                double dx0 = __dsub_rn(qi.x, qj.x);
-                double dx1 = __dsub_rn(qi.y, qj.y);
-                double dr = __dadd_rn(__dmul_rn(dx0, dx0), __dmul_rn(dx1, dx1));
-                if (DIM == 3) {
-                    double dx2 = __dsub_rn(qi.z, qj.z);
-                    dr = __dadd_rn(dr, __dmul_rn(dx2, dx2));
-                }
-                if (!(dr < cut2) || j == s) continue;
-                if (!(__dsqrt_rn(dr) < cut)) continue;
-                uint32_t rcj = D.refcell[j];
-                bool inwin = true;
-                for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
-                if (!inwin) continue;
-                if (cnt < cap) D.nbr[(int64_t)cnt * P.ldn + s] = j;
-                ++cnt;
+    const int W = P.win;
+    const int zlo = DIM == 3 ? max(cz - W, 0) : 0, zhi = DIM == 3 ? min(cz + W, P.nc[2] - 1) : 0;
+    const int ylo = max(cy - W, 0), yhi = min(cy + W, P.nc[1] - 1);
+    const int xlo = max(cx - W, 0), xhi = min(cx + W, P.nc[0] - 1);
+    auto test = [&](const float4 cj) {
+        const float dx = xf - cj.x, dy = yf - cj.y, dz = DIM == 3 ? zf - cj.z : 0.0f;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < hi2)) return;
+        const int j = __float_as_int(cj.w);
+        if (j == s) return;
+        if (!(r2 < lo)) {
+            // rounding band: the reference's test, asymmetricSearch.f90:227-237
+            const double4 qj = D.posh[j];
+            const double dx0 = __dsub_rn(qi.x, qj.x);
+            const double dx1 = __dsub_rn(qi.y, qj.y);
+            double dr = __dadd_rn(__dmul_rn(dx0, dx0), __dmul_rn(dx1, dx1));
+            if (DIM == 3) {
+                const double dx2 = __dsub_rn(qi.z, qj.z);
+                dr = __dadd_rn(dr, __dmul_rn(dx2, dx2));
             }
+            if (!(__dsqrt_rn(dr) < cut)) return;
         }
+        if (need_win) {
+            const uint32_t rcj = D.refcell[j];
+            bool inwin = true;
+            for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
+            if (!inwin) return;
+        }
+        if (cnt < cap) D.nbr[(int64_t)cnt * P.ldn + s] = j;
+        ++cnt;
+    };
+    // rows of cells along x are contiguous in cell order: one range per (y, z);
+    // the next row's range is fetched while the current one is scanned
+    int z = zlo, y = ylo;
+    int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
+    int kb = D.start[row + xlo], ke = D.start[row + xhi + 1];
+    for (;;) {
+        int ny = y + 1, nz = z;
+        if (ny > yhi) { ny = ylo; ++nz; }
+        const bool more = nz <= zhi;
+        int nkb = 0, nke = 0;
+        if (more) {
+            const int64_t nrow = (int64_t)P.nc[0] * ((int64_t)ny + (int64_t)P.nc[1] * nz);
+            nkb = D.start[nrow + xlo];
+            nke = D.start[nrow + xhi + 1];
+        }
+        int k = kb;
+        for (; k + 1 < ke; k += 2) {
+            const float4 c0 = D.cpos[k];
+            const float4 c1 = D.cpos[k + 1];
+            test(c0);
+            test(c1);
+        }
+        if (k < ke) test(D.cpos[k]);
+        if (!more) break;
+        y = ny; z = nz; kb = nkb; ke = nke;
+    }
     D.ncnt[s] = cnt;
     if (cnt > P.nbr_cap && !(mode & 4)) atomicOr(D.err, kErrPairCap);
     atomicMax(D.maxcnt, cnt);
@@ -843,7 +966,13 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-template <int DIM, bool MIXED>
+template <int DIM>
+__device__ __forceinline__ void unpack_cell(uint32_t pk, int& x, int& y, int& z) {
+    if (DIM == 3) { x = (int)(pk & 2047u); y = (int)((pk >> 11) & 2047u); z = (int)((pk >> 22) & 511u); }
+    else { x = (int)(pk & 65535u); y = (int)((pk >> 16) & 32767u); z = 0; }
+}
+
+template <int DIM, bool MIXED, bool PAIR_EXT>
 __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
     using R = typename std::conditional<MIXED, float, double>::type;
     using R4 = typename Vec4<R>::T;
@@ -852,11 +981,24 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
     const int4 mt = D.meta[s];
     if (mt.w >= P.n_real) return;
     const int ti = mt.x;
-    const double4 qi = D.posh[s];
     R4 vi, thi;
-    if (MIXED) { vi = *(const R4*)&D.vmf[s]; thi = *(const R4*)&D.thf[s]; }
-    else { vi = *(const R4*)&D.vm[s]; thi = *(const R4*)&D.th[s]; }
-    const R hi = (R)qi.w;
+    R pix, piy, piz, hi;
+    int cxi = 0, cyi = 0, czi = 0;
+    double4 qi;
+    if (MIXED) {
+        vi = *(const R4*)&D.vmf[s];
+        thi = *(const R4*)&D.thf[s];
+        const float4 pf = D.posf[s];
+        pix = pf.x; piy = pf.y; piz = pf.z;
+        unpack_cell<DIM>(__float_as_uint(pf.w), cxi, cyi, czi);
+        hi = thi.w;
+    } else {
+        vi = *(const R4*)&D.vm[s];
+        thi = *(const R4*)&D.th[s];
+        qi = D.posh[s];
+        hi = (R)qi.w;
+    }
+    const R csf = (R)P.cs;
     const R inv_h = (R)1 / hi;
     // cubic spline (kernel.f90:30-71): grad W = G dx with
     //   G = f/h^2 (-2 + 1.5 q)        q < 1
@@ -869,13 +1011,33 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
     R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
     const int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     const int* __restrict__ nl = D.nbr + s;
+    int jn = n > 0 ? nl[0] : 0;
     for (int k = 0; k < n; ++k) {
-        const int j = nl[(int64_t)k * P.ldn];
-        const double4 qj = D.posh[j];
+        const int j = jn;
+        if (k + 1 < n) jn = nl[(int64_t)(k + 1) * P.ldn];
         R4 vj, thj;
-        if (MIXED) { vj = *(const R4*)&D.vmf[j]; thj = *(const R4*)&D.thf[j]; }
-        else { vj = *(const R4*)&D.vm[j]; thj = *(const R4*)&D.th[j]; }
-        const R dx = (R)(qi.x - qj.x), dy = (R)(qi.y - qj.y), dz = DIM == 3 ? (R)(qi.z - qj.z) : (R)0;
+        R dx, dy, dz, hj, Vj;
+        if (MIXED) {
+            vj = *(const R4*)&D.vmf[j];
+            thj = *(const R4*)&D.thf[j];
+            const float4 pf = D.posf[j];
+            int cxj, cyj, czj;
+            unpack_cell<DIM>(__float_as_uint(pf.w), cxj, cyj, czj);
+            dx = (pix - pf.x) + (R)(cxi - cxj) * csf;
+            dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
+            dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
+            hj = thj.w;
+            Vj = vj.w * fast_rcp(thj.x);
+        } else {
+            vj = *(const R4*)&D.vm[j];
+            thj = *(const R4*)&D.th[j];
+            const double4 qj = D.posh[j];
+            dx = (R)(qi.x - qj.x);
+            dy = (R)(qi.y - qj.y);
+            dz = DIM == 3 ? (R)(qi.z - qj.z) : (R)0;
+            hj = (R)qj.w;
+            Vj = thj.w;
+        }
         const R r2 = dx * dx + dy * dy + dz * dz;
         const R r = fast_sqrt(r2);
         const R q = r * inv_h;
@@ -886,12 +1048,12 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
         const R dvg = dvx * gx + dvy * gy + dvz * gz;
         const R dxg = dx * gx + dy * gy + dz * gz;
         const R mj = vj.w;
-        sv += thj.w * dvg;          // continuity / div v (density.f90:98-101)
-        sx += thj.w * dxg;          // div x (divergence.f90:17-27)
+        sv += Vj * dvg;             // continuity / div v (density.f90:98-101)
+        sx += Vj * dxg;             // div x (divergence.f90:17-27)
         const R aux = thi.y + thj.y;  // p_i/rho_i^2 + p_j/rho_j^2 (internalForce.f90:236)
         // Monaghan viscosity when approaching (artificialViscosity.f90:80-110)
         const R xv = dx * dvx + dy * dvy + dz * dvz;
-        const R hij = (R)0.5 * (hi + (R)qj.w);
+        const R hij = (R)0.5 * (hi + hj);
         const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
         const R Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
                                      fast_rcp((R)0.5 * (thi.x + thj.x))
@@ -902,7 +1064,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
         az -= cf * gz;
         de_in += mj * aux * dvg;
         de_av += (R)0.5 * mj * Pi * dvg;
-        if (P.pair_ext) {
+        if (PAIR_EXT) {
             const int tj = D.meta[j].x;
             R fe = 0;
             if (ti > 0 && tj < 0 && (double)r < P.lj_r0) {
@@ -911,11 +1073,11 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
             }
             if (ti != tj && ti > 0 && tj > 0) {
                 if (ti < 100 && tj < 100) {
-                    R pe = (hi + (R)qj.w) / ((R)2 * r);
+                    R pe = (hi + hj) / ((R)2 * r);
                     if (pe >= (R)1)
                         fe += (R)P.pen_factor * ((R)pow((double)pe, P.pen_n1) - (R)pow((double)pe, P.pen_n2)) / r2;
                 } else if (!(ti > 100 && tj > 100)) {
-                    R et = r / (R)0.75 / ((hi + (R)qj.w) * (R)0.5);
+                    R et = r / (R)0.75 / ((hi + hj) * (R)0.5);
                     R chi = (R)1 - r / (R)P.couple_delta;
                     if (chi < 0 || chi > 1) chi = 0;
                     R fc = 0;
@@ -947,6 +1109,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
     }
     D.div[s] = make_double2(-(double)sx, -(double)sv);
 }
+
+#include "fluid_fast.hpp"
 
 // Leap-frog correct (timeIntegration.f90:178-252) and bounding-box partials
 // of the new positions for the next step's grid.
@@ -1020,16 +1184,30 @@ __global__ __launch_bounds__(kBlock) void k_correct(Params P, Dev D, int first) 
 
 // Neighbour statistics of the last search (print_statistics, nnps.f90:377-416):
 // out[0] total pairs, out[1] max, out[2] min, out[3] targets with none.
-__global__ __launch_bounds__(kBlock) void k_stats(Params P, Dev D, unsigned long long* out) {
+__global__ __launch_bounds__(kBlock) void k_stats(Params P, Dev D, const int* counts,
+                                                  unsigned long long* out) {
     int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= P.n) return;
-    int4 mt = D.meta[s];
-    if (mt.w >= P.n_real) return;
-    int c = (mt.y == -1) ? 0 : D.ncnt[s];
-    atomicAdd(&out[0], (unsigned long long)c);
-    atomicMax(&out[1], (unsigned long long)c);
-    atomicMin(&out[2], (unsigned long long)c);
-    if (c == 0) atomicAdd(&out[3], 1ull);
+    unsigned long long tot = 0, mx = 0, mn = ~0ull, zero = 0;
+    if (s < P.n) {
+        int4 mt = D.meta[s];
+        if (mt.w < P.n_real) {
+            unsigned long long c = (mt.y == -1) ? 0 : (unsigned long long)counts[s];
+            tot = c; mx = c; mn = c; zero = c == 0;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        tot += __shfl_xor(tot, o);
+        zero += __shfl_xor(zero, o);
+        unsigned long long a = __shfl_xor(mx, o), b = __shfl_xor(mn, o);
+        mx = a > mx ? a : mx;
+        mn = b < mn ? b : mn;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], tot);
+        atomicMax(&out[1], mx);
+        atomicMin(&out[2], mn);
+        atomicAdd(&out[3], zero);
+    }
 }
 
 // --------------------------------------------------------------------- host
@@ -1058,6 +1236,7 @@ struct sphgpu_ctx {
     std::vector<void*> allocs;
     int64_t ncell_alloc = 0;
     int64_t nbr_alloc = 0;
+    int64_t vnbr_alloc = 0;
     int nblk = 0;
     int64_t step = 1;
     bool has_state = false;
@@ -1066,6 +1245,8 @@ struct sphgpu_ctx {
     int64_t last_force_launches = 0;
     bool time_force = false;
     bool force_general = false;   // testing switch: use the general force kernel
+    bool use_verlet = true;       // fluid fast path: candidate list + fused pass
+    bool fused_done = false;      // a fused force pass has run since the upload
     std::string err;
     int* h_err = nullptr;   // pinned
 };
@@ -1099,6 +1280,7 @@ static void free_all(sphgpu_ctx* c) {
     c->D = Dev{};
     c->ncell_alloc = 0;
     c->nbr_alloc = 0;
+    c->vnbr_alloc = 0;
 }
 
 static int validate_config(const sphgpu_config* cfg, std::string* why) {
@@ -1140,6 +1322,8 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
     c->capacity = capacity;
     const char* fg = getenv("SPHGPU_FORCE_GENERAL");
     c->force_general = fg && fg[0] == '1';
+    const char* nv = getenv("SPHGPU_NO_VERLET");
+    c->use_verlet = !(nv && nv[0] == '1');
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return SPHGPU_E_HIP;
@@ -1191,9 +1375,9 @@ static bool fluid_fast_path(const sphgpu_ctx* c) {
            !g.summation_density && !g.artificial_stress && !c->force_general;
 }
 
-// Our search grid: cells of cutoff/kWin (slightly enlarged) over the upload
+// Our search grid: cells of cutoff/win (slightly enlarged) over the upload
 // bounding box plus a margin; keys outside are clamped to the border cells,
-// which keeps every neighbour within the +-kWin sweep.
+// which keeps every neighbour within the +-win sweep.
 static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
     const int dim = c->dim;
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, hmax = 0;
@@ -1206,20 +1390,40 @@ static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
         }
         hmax = std::max(hmax, h[i]);
     }
-    double cs = hmax * c->P.scale_k / kWin * (1.0 + 1e-9);
+    const char* wenv = getenv("SPHGPU_WIN");
+    c->P.win = (wenv && wenv[0] >= '1' && wenv[0] <= '4') ? wenv[0] - '0' : kWinDefault;
+    const char* senv = getenv("SPHGPU_SKIN");
+    const double skin_frac = senv ? atof(senv) : 0.15;
+    const double cut_max = hmax * c->P.scale_k;
+    c->P.skin = (fluid_fast_path(c) && c->use_verlet) ? skin_frac * cut_max : 0.0;
+    c->P.hmax = hmax;
+    const double half = 0.5 * c->P.skin * (1.0 - 1e-9);
+    c->P.disp_thr2 = half * half;
+    double cs = (cut_max + c->P.skin) / c->P.win * (1.0 + 1e-6);
     if (!(cs > 0)) cs = 1.0;
     double lo[3], ext[3];
     for (int d = 0; d < dim; ++d) {
         double span = mx[d] - mn[d];
-        double margin = 0.1 * span + 2 * cs;
+        double margin = 0.05 * span + 2 * cs;
         lo[d] = mn[d] - margin;
         ext[d] = span + 2 * margin;
     }
     const int64_t budget = std::max<int64_t>(8 * c->n, 4096);
+    // per-axis limits of the packed cell coordinates (11/11/9 bits in 3D, 16/15
+    // in 2D; bit 31 marks a particle outside the grid)
+    const int64_t lim[3] = {dim == 3 ? 2047 : 65535, dim == 3 ? 2047 : 32767, 511};
     for (;;) {
+        // the cell size is a float value, so FP32 cell offsets reconstruct pair
+        // separations exactly in the cell part
+        cs = (double)(float)cs;
         int64_t cells = 1;
-        for (int d = 0; d < dim; ++d) cells *= (int64_t)std::ceil(ext[d] / cs) + 1;
-        if (cells <= budget && cells < (1LL << 31) - 2) break;
+        bool ok = true;
+        for (int d = 0; d < dim; ++d) {
+            int64_t nd = (int64_t)std::ceil(ext[d] / cs) + 1;
+            cells *= nd;
+            ok &= nd <= lim[d];
+        }
+        if (ok && cells <= budget && cells < (1LL << 31) - 2) break;
         cs *= 1.25;
     }
     Params& P = c->P;
@@ -1229,24 +1433,100 @@ static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
         P.nc[d] = (int)std::ceil(ext[d] / cs) + 1;
     }
     P.inv_cs = 1.0 / cs;
+    P.cs = cs;
     P.ncell = (int64_t)P.nc[0] * P.nc[1] * P.nc[2];
 }
 
 template <int DIM>
-static void launch_sort(sphgpu_ctx* c) {
+static void launch_sort(sphgpu_ctx* c, Gate G = Gate{nullptr}, int with_ref = 1) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const int nbs = std::min(nb, kStrideBlocks);
+    hipLaunchKernelGGL(k_keys<DIM>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, with_ref);
+    const int m = (int)P.ncell + 1;
+    const int sb = (m + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum, G);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb, G);
+    hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
+                       c->D.start, (int)P.n, G);
+    hipLaunchKernelGGL(k_scatter, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G);
+    hipLaunchKernelGGL(k_cellsort, dim3(std::min((m + kBlock - 1) / kBlock, kStrideBlocks)), dim3(kBlock),
+                       0, c->stream, c->D.start, c->D.perm, m, G);
+    hipLaunchKernelGGL(k_cellpos, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G);
+}
+
+static Gate fast_gate(const sphgpu_ctx* c) { return Gate{c->D.vflag}; }
+
+// The fluid fast path. Head: reference grid, predict + EOS + displacement
+// check. Tail: candidate-list rebuild when the gate is open, fused force pass.
+template <int DIM>
+static void launch_fast_head(sphgpu_ctx* c, int mode) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
-    hipLaunchKernelGGL(k_keys<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-    const int m = (int)P.ncell + 1;
-    const int sb = (m + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb);
-    hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
-                       c->D.start, c->D.fill, (int)P.n);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-    hipLaunchKernelGGL(k_cellsort, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                       c->D.start, c->D.perm, m);
+    if (P.mixed)
+        hipLaunchKernelGGL((k_point_fast<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode);
+    else
+        hipLaunchKernelGGL((k_point_fast<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode);
+}
+
+template <int DIM>
+static void launch_fast_tail(sphgpu_ctx* c, int count_only) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const Gate G = fast_gate(c);
+    launch_sort<DIM>(c, G, 0);
+    hipLaunchKernelGGL(k_verlet_build<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only);
+    if (count_only) return;
+    if (c->time_force) hipEventRecord(c->ev[2], c->stream);
+    if (P.mixed && P.pair_ext)
+        hipLaunchKernelGGL((k_force_fused<DIM, true, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (P.mixed)
+        hipLaunchKernelGGL((k_force_fused<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (P.pair_ext)
+        hipLaunchKernelGGL((k_force_fused<DIM, false, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else
+        hipLaunchKernelGGL((k_force_fused<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (c->time_force) hipEventRecord(c->ev[3], c->stream);
+}
+
+template <int DIM>
+static void launch_fast(sphgpu_ctx* c, int mode, int count_only = 0) {
+    launch_fast_head<DIM>(c, mode);
+    launch_fast_tail<DIM>(c, count_only);
+}
+
+// nsteps leap-frog steps on the fast path: the correct of step n-1 is fused
+// with the predict of step n; the last step ends with a plain correct.
+template <int DIM>
+static void advance_fast(sphgpu_ctx* c, int nsteps) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const int nbs = std::min(nb, kStrideBlocks);
+    launch_fast_head<DIM>(c, c->step == 1 ? 1 : 0);
+    for (int k = 0; k < nsteps; ++k) {
+        c->time_force = (k == nsteps - 1);
+        launch_fast_tail<DIM>(c, 0);
+        const int first = c->step == 1;
+        if (k + 1 < nsteps) {
+            if (P.mixed)
+                hipLaunchKernelGGL((k_correct_predict<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+            else
+                hipLaunchKernelGGL((k_correct_predict<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+            hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
+            hipLaunchKernelGGL(k_refcell<DIM>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D);
+        } else if (P.mixed) {
+            hipLaunchKernelGGL((k_correct<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+        } else {
+            hipLaunchKernelGGL((k_correct<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+        }
+        c->step++;
+    }
+    c->time_force = false;
+}
+
+static void force_rebuild(sphgpu_ctx* c) {
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, c->stream, c->D.vflag);
 }
 
 // One evaluation of the derivatives (mode bits as k_point_build).
@@ -1263,9 +1543,14 @@ template <int DIM>
 static void launch_derivs(sphgpu_ctx* c, int mode) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const bool fast = fluid_fast_path(c);
+    if (fast && c->use_verlet) {
+        launch_fast<DIM>(c, mode);
+        return;
+    }
+    hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, mode);
-    const bool fast = fluid_fast_path(c);
     if (!fast) {
         if (P.kgc) hipLaunchKernelGGL(k_kgc<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
         if (P.summation_density) {
@@ -1278,10 +1563,14 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
             hipLaunchKernelGGL(k_solid_point<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     }
     if (c->time_force) hipEventRecord(c->ev[2], c->stream);
-    if (P.mixed)
-        hipLaunchKernelGGL((k_force_fluid<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (P.mixed && P.pair_ext)
+        hipLaunchKernelGGL((k_force_fluid<DIM, true, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (P.mixed)
+        hipLaunchKernelGGL((k_force_fluid<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (fast && P.pair_ext)
+        hipLaunchKernelGGL((k_force_fluid<DIM, false, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     else if (fast)
-        hipLaunchKernelGGL((k_force_fluid<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force_fluid<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     else
         hipLaunchKernelGGL(k_force<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     if (c->time_force) hipEventRecord(c->ev[3], c->stream);
@@ -1325,6 +1614,9 @@ static int alloc_state(sphgpu_ctx* c) {
     rc |= dalloc(c, &D.th, N); rc |= dalloc(c, &D.div, N); rc |= dalloc(c, &D.eta, N);
     rc |= dalloc(c, &D.tds, N); rc |= dalloc(c, &D.key, N); rc |= dalloc(c, &D.refcell, N);
     rc |= dalloc(c, &D.perm, N); rc |= dalloc(c, &D.ncnt, N);
+    rc |= dalloc(c, &D.cpos, N);
+    rc |= dalloc(c, &D.xb, N); rc |= dalloc(c, &D.vcnt, N); rc |= dalloc(c, &D.ecnt, N);
+    rc |= dalloc(c, &D.vflag, 2);
     bool need_tensor = c->cfg.viscosity || c->P.has_solid;
     if (need_tensor) rc |= dalloc(c, &D.Ep, T);
     // S, stress and dS/dt always exist (zero for fluids) so they can be read back
@@ -1338,6 +1630,7 @@ static int alloc_state(sphgpu_ctx* c) {
     if (c->cfg.kernel_correction) rc |= dalloc(c, &D.kgcA, T);
     if (c->P.mixed) {
         rc |= dalloc(c, &D.vmf, N); rc |= dalloc(c, &D.thf, N); rc |= dalloc(c, &D.drho, N);
+        rc |= dalloc(c, &D.posf, N);
         rc |= dalloc(c, &D.drho_prev, N); rc |= dalloc(c, &D.dvf, N); rc |= dalloc(c, &D.def, N);
     }
     if (c->cfg.summation_density) rc |= dalloc(c, &D.rho_new, N);
@@ -1365,7 +1658,6 @@ static int alloc_cells(sphgpu_ctx* c) {
     int rc = 0;
     rc |= dalloc(c, &D.count, m + 1);
     rc |= dalloc(c, &D.start, m + 1);
-    rc |= dalloc(c, &D.fill, m + 1);
     rc |= dalloc(c, &D.bsum, (m + kScanTile - 1) / kScanTile + 1);
     if (rc) return SPHGPU_E_HIP;
     c->ncell_alloc = m;
@@ -1382,10 +1674,11 @@ static int alloc_nbr(sphgpu_ctx* c, int cap) {
 
 template <int DIM>
 static int size_neighbour_list(sphgpu_ctx* c) {
-    // count-only pass on the uploaded state, then capacity with headroom
+    // count-only passes on the uploaded state, then capacities with headroom
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, nb);
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, 2 | 4);
     HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1396,7 +1689,25 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     int cap = mx + std::max(8, mx / 2);
     c->P.nbr_cap = cap;
     c->P.ldn = c->capacity;
-    return alloc_nbr(c, cap);
+    e = alloc_nbr(c, cap);
+    if (e) return e;
+    if (fluid_fast_path(c) && c->use_verlet) {
+        HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
+        force_rebuild(c);
+        launch_fast<DIM>(c, 2, 1);
+        HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        e = check_err(c);
+        if (e) return e;
+        int vx = c->h_err[1];
+        int vcap = vx + std::max(8, vx / 2);
+        c->P.verlet_cap = vcap;
+        if ((int64_t)vcap * c->capacity > c->vnbr_alloc) {
+            if (dalloc(c, &c->D.vnbr, (size_t)vcap * c->capacity)) return SPHGPU_E_HIP;
+            c->vnbr_alloc = (int64_t)vcap * c->capacity;
+        }
+    }
+    return 0;
 }
 
 extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, const double* x,
@@ -1446,8 +1757,10 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     if (rc) return rc;
     rc = alloc_cells(c);
     if (rc) return rc;
-    // order the slots by our cell key once, so that the sweep reads nearby memory
-    std::vector<uint64_t> keyed(n);
+    // order the slots once by our linear cell key (x fastest): neighbouring
+    // lanes then hold x-neighbouring particles, so the k-th neighbours of a
+    // wavefront's lanes sit in consecutive slots and the gathers coalesce
+    std::vector<std::pair<uint64_t, int64_t>> keyed(n);
     for (int64_t i = 0; i < n; ++i) {
         int64_t cc[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) {
@@ -1455,8 +1768,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
             t = std::min(std::max(t, 0.0), (double)(c->P.nc[d] - 1));
             cc[d] = (int64_t)t;
         }
-        uint64_t key = (uint64_t)(cc[0] + c->P.nc[0] * (cc[1] + (int64_t)c->P.nc[1] * cc[2]));
-        keyed[i] = (key << 32) | (uint64_t)i;
+        keyed[i] = {(uint64_t)(cc[0] + c->P.nc[0] * (cc[1] + (int64_t)c->P.nc[1] * cc[2])), i};
     }
     std::sort(keyed.begin(), keyed.end());
     std::vector<double4> posh(n), vm(n), disp(n), zero4(n);
@@ -1464,7 +1776,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     std::vector<int4> meta(n);
     std::vector<double> vshear((size_t)n * dim * dim, 0.0);
     for (int64_t s = 0; s < n; ++s) {
-        int64_t i = (int64_t)(keyed[s] & 0xffffffffu);
+        int64_t i = keyed[s].second;
         posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
         double vx = v ? v[i * dim] : 0, vy = v ? v[i * dim + 1] : 0, vz = (v && dim == 3) ? v[i * dim + 2] : 0;
         vm[s] = make_double4(vx, vy, vz, mass[i]);
@@ -1506,6 +1818,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->step = 1;
     c->has_state = true;
+    c->fused_done = false;
     rc = dim == 2 ? size_neighbour_list<2>(c) : size_neighbour_list<3>(c);
     if (rc) { c->has_state = false; return rc; }
     // the grid partials of the uploaded positions seed the first step
@@ -1519,8 +1832,10 @@ extern "C" int sphgpu_derivatives(sphgpu_ctx* c) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    force_rebuild(c);
     if (c->dim == 2) launch_derivs<2>(c, 2);
     else launch_derivs<3>(c, 2);
+    c->fused_done = fluid_fast_path(c) && c->use_verlet;
     HIPCHK(c, hipGetLastError());
     return check_err(c);
 }
@@ -1529,18 +1844,25 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     if (nsteps < 0) return fail(c, SPHGPU_E_ARG, "negative step count");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
-    // positions may have been changed by sphgpu_derivatives' caller: refresh partials
+    // refresh the grid partials and rebuild the candidate list on the first step
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    force_rebuild(c);
+    if (nsteps > 0) c->fused_done = fluid_fast_path(c) && c->use_verlet;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     double force_ms = 0;
     int64_t force_n = 0;
-    for (int k = 0; k < nsteps; ++k) {
-        c->time_force = (k == nsteps - 1);
-        if (c->dim == 2) launch_step<2>(c);
-        else launch_step<3>(c);
-        c->step++;
+    if (fluid_fast_path(c) && c->use_verlet && nsteps > 0) {
+        if (c->dim == 2) advance_fast<2>(c, nsteps);
+        else advance_fast<3>(c, nsteps);
+    } else {
+        for (int k = 0; k < nsteps; ++k) {
+            c->time_force = (k == nsteps - 1);
+            if (c->dim == 2) launch_step<2>(c);
+            else launch_step<3>(c);
+            c->step++;
+        }
+        c->time_force = false;
     }
-    c->time_force = false;
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, hipGetLastError());
     int rc = check_err(c);
@@ -1679,9 +2001,11 @@ extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
     if (c->dim == 2) {
+        hipLaunchKernelGGL(k_grid<2>, dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, nb);
         launch_sort<2>(c);
         launch_point_build<2>(c, 2);
     } else {
+        hipLaunchKernelGGL(k_grid<3>, dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, nb);
         launch_sort<3>(c);
         launch_point_build<3>(c, 2);
     }
@@ -1726,7 +2050,8 @@ extern "C" int sphgpu_neighbour_stats(sphgpu_ctx* c, int64_t* out4) {
     HIPCHK(c, hipMalloc(&d, sizeof(init)));
     HIPCHK(c, hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_stats, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, d);
+    const int* counts = (fluid_fast_path(c) && c->use_verlet && c->fused_done) ? c->D.ecnt : c->D.ncnt;
+    hipLaunchKernelGGL(k_stats, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, counts, d);
     unsigned long long h[4];
     hipError_t e1 = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
     hipError_t e2 = hipStreamSynchronize(c->stream);
