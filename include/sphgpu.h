@@ -195,6 +195,8 @@ int sphgpu_timing(sphgpu_ctx* ctx, double* advance_ms, double* force_kernel_ms,
 /* Number of the step the next sphgpu_advance performs (1 after upload). */
 int64_t sphgpu_step(sphgpu_ctx* ctx);
 
+/* Message of the last failed call on ctx; with ctx == NULL, the message of
+ * the last failed sphgpu_create on the calling thread. */
 const char* sphgpu_last_error(sphgpu_ctx* ctx);
 
 void sphgpu_destroy(sphgpu_ctx* ctx);

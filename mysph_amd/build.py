@@ -1,4 +1,6 @@
-"""Builds libsphgpu.so (the HIP kernels + C ABI) in-tree for gfx950."""
+"""Builds, in-tree: libsphgpu.so (the HIP kernels + C ABI) for gfx950, and
+the Fortran host program bin/mysph_host (amdflang) that drives it through
+the iso_c_binding module fortran/sphgpu_m.f90."""
 from __future__ import annotations
 
 import os
@@ -10,6 +12,12 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsphgpu.so")
 SOURCES = [os.path.join(HERE, "csrc", "sphgpu.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(HERE, "_build", "fortran")
+HOST = os.path.join(HERE, "bin", "mysph_host")
+# module dependency order
+HOST_SOURCES = ["sphgpu_m.f90", "toml_lite_m.f90", "cases_m.f90", "output_m.f90", "mysph_host.f90"]
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -43,5 +51,36 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the Fortran host against libsphgpu.so (build() first)."""
+    if not os.path.exists(LIB):
+        raise RuntimeError("libsphgpu.so is not built")
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(HOST), exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(FORTRAN_BUILD, src.replace(".f90", ".o"))
+        cmd = [FC, "-cpp", "-O2", "-module-dir", FORTRAN_BUILD, "-c",
+               os.path.join(FORTRAN_DIR, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"amdflang failed on {src}")
+        objs.append(obj)
+    tmp = HOST + ".tmp"
+    cmd = [FC, *objs, "-L" + HERE, "-lsphgpu", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("linking mysph_host failed")
+    os.replace(tmp, HOST)
+    return HOST
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -48,9 +48,13 @@ def jitter_unit(ids: np.ndarray, dim: int, seed: int = SEED_JITTER) -> np.ndarra
 
 
 def _tait_inverse(p, B, rho0=1000.0, gamma=7.0):
-    """rho = (p/B + 1)**(1/gamma) * rho0, eos.f90:46-47 (libm pow per value)."""
+    """rho = (p/B + 1)**(1/gamma) * rho0, eos.f90:46-47, with the C library's
+    pow per value: numpy's vectorised power may take a SIMD path whose last
+    bit differs from libm's and from one host CPU to another, while libm pow
+    is what the Fortran host and the reference call."""
     e = 1.0 / gamma
-    return np.array([math.pow(pi / B + 1.0, e) * rho0 for pi in np.asarray(p, dtype=np.float64)])
+    base = np.asarray(p, dtype=np.float64) / B + 1.0
+    return np.fromiter((math.pow(b, e) for b in base), dtype=np.float64, count=base.size) * rho0
 
 
 def _empty(n: int, dim: int) -> dict:
@@ -226,7 +230,7 @@ def dam_break(dim: int, dx: float, column=(1.0, 2.0), tank=(4.0, 3.0), layers: i
     fl["x"][:] = xf
     fl["mass"][:] = 1000.0 * dx ** dim
     fl["p"][:] = 1000.0 * g * np.maximum(H - xf[:, -1], 0.0)
-    fl["rho"][:] = np.power(fl["p"] / B + 1.0, 1.0 / 7.0) * 1000.0
+    fl["rho"][:] = _tait_inverse(fl["p"], B)
     fl["c"][:] = c0
     fl["h"][:] = h
     fl["type"][:] = 2
@@ -247,7 +251,7 @@ def dam_break(dim: int, dx: float, column=(1.0, 2.0), tank=(4.0, 3.0), layers: i
     bd["x"][:] = xb
     bd["mass"][:] = 1000.0 * dx ** dim
     bd["p"][:] = 1000.0 * g * np.maximum(H - xb[:, -1], 0.0)
-    bd["rho"][:] = np.power(bd["p"] / B + 1.0, 1.0 / 7.0) * 1000.0
+    bd["rho"][:] = _tait_inverse(bd["p"], B)
     bd["c"][:] = c0
     bd["h"][:] = h
     bd["type"][:] = -2

@@ -1246,7 +1246,7 @@ struct sphgpu_ctx {
     bool time_force = false;
     bool force_general = false;   // testing switch: use the general force kernel
     bool use_verlet = true;       // fluid fast path: candidate list + fused pass
-    bool fused_done = false;      // a fused force pass has run since the upload
+    bool fused_done = false;      // a fused pass has run since the upload: candidate list valid
     std::string err;
     int* h_err = nullptr;   // pinned
 };
@@ -1305,18 +1305,34 @@ static int validate_config(const sphgpu_config* cfg, std::string* why) {
 
 extern "C" int sphgpu_abi_version(void) { return SPHGPU_ABI_VERSION; }
 
+// Message of the last failed sphgpu_create on this thread (there is no
+// context to hold it); sphgpu_last_error(NULL) returns it.
+static thread_local std::string g_create_error;
+
+static int create_fail(sphgpu_ctx* c, int rc, const std::string& why) {
+    g_create_error = "sphgpu_create: " + why;
+    if (c) {
+        for (auto& e : c->ev)
+            if (e) hipEventDestroy(e);
+        if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+    }
+    return rc;
+}
+
+static std::string hip_why(const char* call, hipError_t e) {
+    return std::string(call) + " failed: " + hipGetErrorString(e);
+}
+
 extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_ctx** out) {
-    if (!cfg || !out || capacity <= 0) return SPHGPU_E_ARG;
+    if (!out) return create_fail(nullptr, SPHGPU_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!cfg || capacity <= 0) return create_fail(nullptr, SPHGPU_E_ARG, "NULL config or capacity <= 0");
     std::string why;
     int rc = validate_config(cfg, &why);
-    *out = nullptr;
+    if (rc) return create_fail(nullptr, rc, why);
     sphgpu_ctx* c = new (std::nothrow) sphgpu_ctx();
-    if (!c) return SPHGPU_E_ARG;
-    if (rc) {
-        fprintf(stderr, "sphgpu_create: %s\n", why.c_str());
-        delete c;
-        return rc;
-    }
+    if (!c) return create_fail(nullptr, SPHGPU_E_ARG, "out of host memory");
     c->cfg = *cfg;
     c->dim = cfg->dim;
     c->capacity = capacity;
@@ -1324,15 +1340,22 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
     c->force_general = fg && fg[0] == '1';
     const char* nv = getenv("SPHGPU_NO_VERLET");
     c->use_verlet = !(nv && nv[0] == '1');
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return SPHGPU_E_HIP;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        c->stream = nullptr;
+        return create_fail(c, SPHGPU_E_HIP, hip_why("hipStreamCreateWithFlags", e));
     }
-    for (auto& e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) { delete c; return SPHGPU_E_HIP; }
-    if (hipHostMalloc((void**)&c->h_err, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) {
-        delete c;
-        return SPHGPU_E_HIP;
+    for (auto& ev : c->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+            ev = nullptr;
+            return create_fail(c, SPHGPU_E_HIP, hip_why("hipEventCreate", e));
+        }
+    }
+    e = hipHostMalloc((void**)&c->h_err, sizeof(int) * 4, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->h_err = nullptr;
+        return create_fail(c, SPHGPU_E_HIP, hip_why("hipHostMalloc", e));
     }
     *out = c;
     return 0;
@@ -1832,7 +1855,10 @@ extern "C" int sphgpu_derivatives(sphgpu_ctx* c) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
-    force_rebuild(c);
+    // the candidate list of the fast path survives between calls (the
+    // displacement vote reopens it), so the result does not depend on how
+    // the host splits its calls
+    if (!c->fused_done) force_rebuild(c);
     if (c->dim == 2) launch_derivs<2>(c, 2);
     else launch_derivs<3>(c, 2);
     c->fused_done = fluid_fast_path(c) && c->use_verlet;
@@ -1844,9 +1870,9 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     if (nsteps < 0) return fail(c, SPHGPU_E_ARG, "negative step count");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
-    // refresh the grid partials and rebuild the candidate list on the first step
+    // refresh the grid partials; build the candidate list if none is valid
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
-    force_rebuild(c);
+    if (!c->fused_done) force_rebuild(c);
     if (nsteps > 0) c->fused_done = fluid_fast_path(c) && c->use_verlet;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     double force_ms = 0;
@@ -2086,7 +2112,10 @@ extern "C" int sphgpu_timing(sphgpu_ctx* c, double* advance_ms, double* force_ke
 
 extern "C" int64_t sphgpu_step(sphgpu_ctx* c) { return c ? c->step : -1; }
 
-extern "C" const char* sphgpu_last_error(sphgpu_ctx* c) { return c ? c->err.c_str() : "null context"; }
+extern "C" const char* sphgpu_last_error(sphgpu_ctx* c) {
+    if (c) return c->err.c_str();
+    return g_create_error.empty() ? "no context" : g_create_error.c_str();
+}
 
 extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     if (!c) return;

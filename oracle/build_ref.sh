@@ -41,3 +41,21 @@ done
 "$FC" -cpp -fopenmp -O2 -w -I"$OUT/obj" -module-dir "$OUT/obj" -c "$HERE/ref_driver.f90" -o "$OUT/obj/ref_driver.o"
 "$FC" -fopenmp -O2 $OBJS "$OUT/obj/ref_driver.o" -o "$OUT/ref_driver"
 echo "build_ref: built $OUT/ref_driver"
+
+# Drop-in link: the same reference program with single_step taken from
+# mysph_amd/fortran/single_step_gpu.f90 instead of singleStep.f90, so the
+# reference's own leap_frog drives the GPU derivatives through libsphgpu.
+REPO="$(cd "$HERE/.." && pwd)"
+if [ -f "$REPO/mysph_amd/libsphgpu.so" ]; then
+    F="$REPO/mysph_amd/fortran"
+    "$FC" -cpp -O2 -module-dir "$OUT/obj" -c "$F/sphgpu_m.f90" -o "$OUT/obj/sphgpu_m.o"
+    "$FC" -cpp -fopenmp -O2 -w -I"$OUT/obj" -module-dir "$OUT/obj" -c "$F/single_step_gpu.f90" \
+        -o "$OUT/obj/single_step_gpu.o"
+    GOBJS=""
+    for o in $OBJS; do
+        [ "$(basename "$o")" = "singleStep.o" ] || GOBJS="$GOBJS $o"
+    done
+    "$FC" -fopenmp -O2 $GOBJS "$OUT/obj/ref_driver.o" "$OUT/obj/sphgpu_m.o" "$OUT/obj/single_step_gpu.o" \
+        -L"$REPO/mysph_amd" -lsphgpu -Wl,-rpath,'$ORIGIN/../../mysph_amd' -o "$OUT/ref_driver_gpu"
+    echo "build_ref: built $OUT/ref_driver_gpu"
+fi

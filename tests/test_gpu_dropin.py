@@ -1,0 +1,76 @@
+"""Drop-in parity: the reference's own program (case set-up, dummy particles,
+leap_frog) with its single_step replaced at link time by
+mysph_amd/fortran/single_step_gpu.f90, against the same program with the
+reference's single_step (src/TimeIntegration/singleStep.f90).
+
+Both binaries are built by oracle/build_ref.sh from the reference sources;
+they differ only in the object that defines single_step. Bars as for the
+other FP64 parity tests: 1e-12 for one derivative evaluation, 1e-10 after
+one and ten steps, 1e-8 after 100 (norm-relative per field).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.parity import assert_close
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+GPU = os.path.join(ROOT, "oracle", "_ref", "ref_driver_gpu")
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(GPU)),
+                       reason="oracle/_ref drivers not built (needs the reference sources at build time)"),
+]
+
+STATE = ("x", "v", "rho", "p", "e")
+
+
+def run(exe, case, mode, nsteps, tmp_path, threads=8):
+    out = tmp_path / f"{os.path.basename(exe)}_{case}_{mode}_{nsteps}.bin"
+    env = dict(os.environ, REF_THREADS=str(threads))
+    print(f"run {os.path.basename(exe)} {case} {mode} {nsteps}", flush=True)
+    r = subprocess.run([exe, case, mode, str(nsteps), str(out)], env=env, cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, f"{exe} failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
+    return oracle.read_dump(str(out))
+
+
+def nreal(d, k, comps=1):
+    return d[k][: int(d["ntotal"][0]) * comps]
+
+
+@pytest.mark.parametrize("case", ["dam_break", "can_beam", "can_beam_3d"])
+def test_dropin_derivatives(case, tmp_path):
+    ref = run(REF, case, "deriv", 1, tmp_path)
+    gpu = run(GPU, case, "deriv", 1, tmp_path)
+    dim = int(ref["dim"][0])
+    assert int(gpu["ntotal"][0]) == int(ref["ntotal"][0])
+    assert int(gpu["ndummy"][0]) == int(ref["ndummy"][0])
+    for k in ("dvdt", "drhodt", "dedt"):
+        assert_close(f"{case} {k}", gpu[k], ref[k], 1e-12)
+    if case != "dam_break":
+        assert_close(f"{case} dsdt", gpu["dsdt"], ref["dsdt"], 1e-12)
+        assert_close(f"{case} stress", nreal(gpu, "stress", dim * dim), nreal(ref, "stress", dim * dim), 1e-12)
+    assert_close(f"{case} p", gpu["p"], ref["p"], 1e-12)
+    assert_close(f"{case} divv", nreal(gpu, "divv"), nreal(ref, "divv"), 1e-12)
+    # the state the reference does not change is passed through untouched
+    for k in ("x", "v", "mass", "h", "type"):
+        assert np.array_equal(gpu[k], ref[k]), k
+
+
+@pytest.mark.parametrize("case,nsteps,tol", [("dam_break", 1, 1e-10), ("dam_break", 10, 1e-10),
+                                             ("dam_break", 100, 1e-8), ("can_beam", 200, 1e-8)])
+def test_dropin_leap_frog(case, nsteps, tol, tmp_path):
+    ref = run(REF, case, "run", nsteps, tmp_path)
+    gpu = run(GPU, case, "run", nsteps, tmp_path)
+    dim = int(ref["dim"][0])
+    for k in STATE:
+        comps = dim if k in ("x", "v") else 1
+        assert_close(f"{case} {nsteps} {k}", nreal(gpu, k, comps), nreal(ref, k, comps), tol)
