@@ -13,29 +13,34 @@
 
 // Predict + EOS + pair inputs for every particle, the reference cell (for the
 // window check), and the largest displacement since the last list build.
-template <int DIM, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_point_fast(Params P, Dev D, int mode) {
-    // mode bit 0: first step (no predict); bit 1: no predict at all
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+template <int DIM, bool MIXED, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_point_fast(Params P0, Dev D, int mode, RunMap M) {
+    // mode bit 0: first step (no predict); bit 1: no predict at all. In a
+    // batch the first step is per run (its step0).
+    int lb;
+    const Params& P = block_run<BATCH>(P0, M, blockIdx.x, lb);
+    const bool first = BATCH ? P.step0 == 1 : (mode & 1) != 0;
+    const int l = lb * kBlock + threadIdx.x;
+    const int s = P.s0 + l;
     bool moved = false;
-    if (s < P.n) {
+    if (l < P.n) {
         const int4 mt = D.meta[s];
         const bool real = mt.w < P.n_real;
         if (MIXED) {
-            if (real && !(mode & 2)) predict_mixed<DIM>(P, D, s, mode & 1);
+            if (real && !(mode & 2)) predict_mixed<DIM>(P, D, s, first);
             eos_mixed(P, D, s, mt.x);
         } else {
-            if (real && !(mode & 2)) predict_one<DIM>(P, D, s, mt, mode & 1);
+            if (real && !(mode & 2)) predict_one<DIM>(P, D, s, mt, first);
             eos_th(P, D, s, mt);
         }
         const double4 q = D.posh[s];
-        if (mt.y != -1 && D.grid->need_win) {
-            const RefGrid g = *D.grid;
+        if (mt.y != -1 && P.grid->need_win) {
+            const RefGrid g = *P.grid;
             bool bad = false;
             const double xs[3] = {q.x, q.y, q.z};
             int rc[3] = {1, 1, 1};
             for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
-            if (bad) atomicOr(D.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+            if (bad) atomicOr(P.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
                                                                                       : kErrNonFinite);
             D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
         }
@@ -45,17 +50,21 @@ __global__ __launch_bounds__(kBlock) void k_point_fast(Params P, Dev D, int mode
         moved = !(d2 < P.disp_thr2);   // NaN also forces a rebuild
     }
     // one atomic per wave, and only when some lane crossed the threshold
-    if (__any(moved) && (threadIdx.x & 63) == 0 && __atomic_load_n(&D.vflag[1], __ATOMIC_RELAXED) == 0ull)
-        atomicOr(&D.vflag[1], 1ull);
+    vote_rebuild(P, moved);
 }
 
 // Candidate list: every j within cut_i + skin of i (superset is harmless), from
 // the +-win sweep over the cell-ordered FP32 positions. Also records the
 // build positions of all particles.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_verlet_build(Params P, Dev D, Gate G, int count_only) {
-    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= P.n || !gate_open(G)) return;
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate G0, int count_only,
+                                                         RunMap RM) {
+    int lb;
+    const Params& P = block_run<BATCH>(P0, RM, xcd_block(blockIdx.x, gridDim.x), lb);
+    const Gate G = BATCH ? Gate{P.vflag} : G0;
+    const int l = lb * kBlock + threadIdx.x;
+    const int s = P.s0 + l;
+    if (l >= P.n || !gate_open(G)) return;
     const double4 qi = D.posh[s];
     D.xb[s] = qi;
     const int4 mt = D.meta[s];
@@ -65,13 +74,14 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P, Dev D, Gate G
         return;
     }
     const double cutv = qi.w * (double)P.scale_k + P.skin;
-    const RefGrid g = *D.grid;
+    const RefGrid g = *P.grid;
     const double u = 5.9604644775390625e-8;
     const double e = g.pos_err + u * cutv;
     const double M = 2.0 * 1.7320508075688772 * cutv * e + 3.0 * e * e + 3.0 * u * cutv * cutv;
     const float lim = (float)(cutv * cutv + 2.0 * M + 1e-6 * cutv * cutv);
     const float xf = (float)(qi.x - P.org[0]), yf = (float)(qi.y - P.org[1]), zf = (float)(qi.z - P.org[2]);
-    const uint32_t key = D.key[s];
+    const uint32_t key = D.key[s] - (uint32_t)P.c0;   // cell within the run's grid
+    const int* __restrict__ start = D.start + P.c0;
     const int cx = (int)(key % (uint32_t)P.nc[0]);
     const int rest = (int)(key / (uint32_t)P.nc[0]);
     const int cy = rest % P.nc[1];
@@ -93,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P, Dev D, Gate G
     };
     int z = zlo, y = ylo;
     int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
-    int kb = D.start[row + xlo], ke = D.start[row + xhi + 1];
+    int kb = start[row + xlo], ke = start[row + xhi + 1];
     for (;;) {
         int ny = y + 1, nz = z;
         if (ny > yhi) { ny = ylo; ++nz; }
@@ -101,8 +111,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P, Dev D, Gate G
         int nkb = 0, nke = 0;
         if (more) {
             const int64_t nrow = (int64_t)P.nc[0] * ((int64_t)ny + (int64_t)P.nc[1] * nz);
-            nkb = D.start[nrow + xlo];
-            nke = D.start[nrow + xhi + 1];
+            nkb = start[nrow + xlo];
+            nke = start[nrow + xhi + 1];
         }
         int k = kb;
         for (; k + 1 < ke; k += 2) {
@@ -116,8 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P, Dev D, Gate G
         y = ny; z = nz; kb = nkb; ke = nke;
     }
     D.vcnt[s] = cnt;
-    if (!count_only && cnt > P.verlet_cap) atomicOr(D.err, kErrPairCap);
-    atomicMax(D.maxcnt, cnt);
+    if (!count_only && cnt > P.verlet_cap) atomicOr(P.err, kErrPairCap);
+    atomicMax(P.maxcnt, cnt);
 }
 
 // Exact neighbour test of one candidate in FP64, the reference's arithmetic:
@@ -140,17 +150,26 @@ __device__ __forceinline__ double exact_dr(const double4& qi, const double4& qj,
 
 // Fused pass over the candidate list: exact neighbour test + window, then the
 // same physics as k_force_fluid for the accepted pairs.
-template <int DIM, bool MIXED, bool PAIR_EXT>
-__global__ __launch_bounds__(kBlock) void k_force_fused(Params P, Dev D) {
+template <int DIM, bool MIXED, bool PAIR_EXT, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap RM) {
     using R = typename std::conditional<MIXED, float, double>::type;
     using R4 = typename Vec4<R>::T;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // every gated kernel of this step has run: clear the gate for the next
-        D.vflag[0] = 0ull;
-        D.vflag[1] = 0ull;
+    int lb;
+    const Params& P = block_run<BATCH>(P0, RM, xcd_block(blockIdx.x, gridDim.x), lb);
+    if (threadIdx.x == 0) {
+        // every gated kernel of this step has run: clear the gates for the next
+        if (lb == 0) {
+            P.vflag[0] = 0ull;
+            P.vflag[1] = 0ull;
+        }
+        if (BATCH && blockIdx.x == 0) {
+            P.vany[0] = 0ull;
+            P.vany[1] = 0ull;
+        }
     }
-    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= P.n) return;
+    const int l = lb * kBlock + threadIdx.x;
+    const int s = P.s0 + l;
+    if (l >= P.n) return;
     const int4 mt = D.meta[s];
     if (mt.w >= P.n_real) return;
     const int ti = mt.x;
@@ -183,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P, Dev D) {
     }
     const R hi = (R)qi.w;
     // reference window (only when it can exclude a pair of this particle)
-    const RefGrid g = *D.grid;
+    const RefGrid g = *P.grid;
     int wr[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
     bool need_win = false;
     const uint32_t rci = g.need_win ? D.refcell[s] : 0u;
@@ -328,12 +347,17 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P, Dev D) {
 // displacement check, for real particles; bounding-box partials of the new
 // positions for the grid of step n. Dummies are static: their pair inputs
 // were written by k_point_fast and do not change.
-template <int DIM, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_correct_predict(Params P, Dev D, int first) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+template <int DIM, bool MIXED, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, int first_arg, RunMap M,
+                                                            int kstep) {
+    int lb;
+    const Params& P = block_run<BATCH>(P0, M, blockIdx.x, lb);
+    const bool first = BATCH ? (P.step0 + kstep == 1) : first_arg != 0;
+    const int l = lb * kBlock + threadIdx.x;
+    const int s = P.s0 + l;
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     bool moved = false;
-    if (s < P.n) {
+    if (l < P.n) {
         const int4 mt = D.meta[s];
         double4 q = D.posh[s];
         if (mt.w < P.n_real) {
@@ -387,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P, Dev D, int
             if (DIM == 3) v.z = v.z + h2 * dvz;
             D.vm[s] = v;
             if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
-                atomicOr(D.err, kErrNonFinite);
+                atomicOr(P.err, kErrNonFinite);
             if (MIXED) eos_mixed(P, D, s, mt.x);
             else eos_th(P, D, s, mt);
             const double4 b = D.xb[s];
@@ -396,18 +420,21 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P, Dev D, int
         }
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0 && __atomic_load_n(&D.vflag[1], __ATOMIC_RELAXED) == 0ull)
-        atomicOr(&D.vflag[1], 1ull);
-    bbox_block_reduce(mn, mx, D.partial, blockIdx.x);
+    vote_rebuild(P, moved);
+    bbox_block_reduce(mn, mx, P.partial, lb);
 }
 
 // Reference cells of all particles, needed only when the reference window can
 // exclude pairs (grid flag); otherwise the launch returns at once.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_refcell(Params P, Dev D) {
-    if (!D.grid->need_win) return;
-    const RefGrid g = *D.grid;
-    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_refcell(Params P0, Dev D, RunMap M) {
+    if (!BATCH && !P0.grid->need_win) return;
+    const int lim = BATCH ? M.nslot : P0.n;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
+        const Params* Pp = slot_run<BATCH>(P0, M, s);
+        if (!Pp || !Pp->grid->need_win) continue;
+        const Params& P = *Pp;
+        const RefGrid g = *P.grid;
         const int4 mt = D.meta[s];
         if (mt.y == -1) continue;
         const double4 q = D.posh[s];
@@ -415,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_refcell(Params P, Dev D) {
         const double xs[3] = {q.x, q.y, q.z};
         int rc[3] = {1, 1, 1};
         for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
-        if (bad) atomicOr(D.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+        if (bad) atomicOr(P.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
                                                                                   : kErrNonFinite);
         D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
     }

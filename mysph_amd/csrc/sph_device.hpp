@@ -56,6 +56,19 @@ struct Params {
     double disp_thr2;   // (skin/2)^2, slightly reduced
     int verlet_cap;
     double hmax;        // largest smoothing length
+    // where this run lives: a single-run context has s0 = 0, c0 = 0; a run of
+    // a batch owns slots [s0, s0 + n) and cells [c0, c0 + ncell] of the
+    // shared arena, and its own grid, gate and error words
+    int s0;
+    int64_t c0;
+    int nblk;                   // blocks of kBlock slots the run spans
+    int64_t step0;              // number of the first step of the launch sequence (batch)
+    struct RefGrid* grid;
+    unsigned long long* vflag;  // the run's rebuild gate (see Gate)
+    unsigned long long* vany;   // the batch-wide gate of the shared sort (== vflag for one run)
+    int* err;
+    int* maxcnt;
+    double* partial;            // 6 doubles of bounding-box partials per block of the run
 };
 
 // Rebuild gate of the candidate list: flag[0] forces a rebuild, flag[1] is set
@@ -76,6 +89,49 @@ struct RefGrid {
     int need_win;     // the reference window can exclude a pair (else it need not be checked)
     double pos_err;   // bound on |fl32(x - org) - (x - org)| over all particles
 };
+
+// A batch keeps many independent runs in one arena and covers all of them
+// with each launch. Per-particle kernels map a launch block to (run row, block
+// within the run); slot-strided kernels look the run of a slot up.
+struct RunMap {
+    const Params* runs;   // device run table (batch launches only)
+    const int2* blk;      // launch block -> (run row, block within the run)
+    const int* runof;     // slot -> run row; -1 marks a free slot
+    const int* rows;      // rows of the active runs (one block each in k_grid)
+    int nslot;            // slots below the arena's high-water mark
+    int nact;             // particles of the active runs (entries of the cell order)
+};
+
+template <bool BATCH>
+__device__ __forceinline__ const Params& block_run(const Params& P0, const RunMap& M, int b, int& lb) {
+    if constexpr (BATCH) {
+        const int2 e = M.blk[b];
+        lb = e.y;
+        return M.runs[e.x];
+    } else {
+        lb = b;
+        return P0;
+    }
+}
+
+template <bool BATCH>
+__device__ __forceinline__ const Params* slot_run(const Params& P0, const RunMap& M, int s) {
+    if constexpr (BATCH) {
+        const int r = M.runof[s];
+        return r < 0 ? nullptr : &M.runs[r];
+    } else {
+        return &P0;
+    }
+}
+
+// Set the gate bit of a run (and of the batch) once per wave.
+__device__ __forceinline__ void vote_rebuild(const Params& P, bool moved) {
+    if (__any(moved) && (threadIdx.x & 63) == 0) {
+        if (__atomic_load_n(&P.vflag[1], __ATOMIC_RELAXED) == 0ull) atomicOr(&P.vflag[1], 1ull);
+        if (P.vany != P.vflag && __atomic_load_n(&P.vany[1], __ATOMIC_RELAXED) == 0ull)
+            atomicOr(&P.vany[1], 1ull);
+    }
+}
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {
     // Work-groups are dispatched round-robin over the XCDs; remap so that each

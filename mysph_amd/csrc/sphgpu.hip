@@ -121,26 +121,30 @@ __device__ __forceinline__ void bbox_block_reduce(double mn[3], double mx[3], do
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bbox(Params P, Dev D) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_bbox(Params P0, Dev D, RunMap M) {
+    int lb;
+    const Params& P = block_run<BATCH>(P0, M, blockIdx.x, lb);
+    const int l = lb * kBlock + threadIdx.x;
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (s < P.n) {
-        double4 q = D.posh[s];
+    if (l < P.n) {
+        double4 q = D.posh[P.s0 + l];
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
-        if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z)) atomicOr(D.err, kErrNonFinite);
+        if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z)) atomicOr(P.err, kErrNonFinite);
     }
-    bbox_block_reduce(mn, mx, D.partial, blockIdx.x);
+    bbox_block_reduce(mn, mx, P.partial, lb);
 }
 
-// Reference grid of this step, asymmetricSearch.f90:49-131. One block.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_grid(Params P, Dev D, int nparts) {
+// Reference grid of this step, asymmetricSearch.f90:49-131. One block per run.
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_grid(Params P0, Dev D, RunMap M) {
+    const Params& P = BATCH ? M.runs[M.rows[blockIdx.x]] : P0;
     __shared__ double smn[3][kBlock], smx[3][kBlock];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = threadIdx.x; b < nparts; b += kBlock)
+    for (int b = threadIdx.x; b < P.nblk; b += kBlock)
         for (int d = 0; d < 3; ++d) {
-            mn[d] = fmin(mn[d], D.partial[b * 6 + d]);
-            mx[d] = fmax(mx[d], D.partial[b * 6 + 3 + d]);
+            mn[d] = fmin(mn[d], P.partial[b * 6 + d]);
+            mx[d] = fmax(mx[d], P.partial[b * 6 + 3 + d]);
         }
     for (int d = 0; d < 3; ++d) { smn[d][threadIdx.x] = mn[d]; smx[d][threadIdx.x] = mx[d]; }
     __syncthreads();
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_grid(Params P, Dev D, int nparts) {
         // reference): report it and keep the grid usable
         for (int d = 0; d < DIM; ++d)
             if (!(delta[d] > 0.0) || g.num[d] < 1) {
-                atomicOr(D.err, kErrOutOfRange);
+                atomicOr(P.err, kErrOutOfRange);
                 g.num[d] = 1;
                 g.maxc[d] = g.minc[d] + 1.0;
             }
@@ -202,15 +206,19 @@ __global__ __launch_bounds__(kBlock) void k_grid(Params P, Dev D, int nparts) {
         for (int d = 0; d < DIM; ++d)
             X = fmax(X, fmax(fabs(smx[d][0] - P.org[d]), fabs(smn[d][0] - P.org[d])));
         g.pos_err = 2.0 * 5.9604644775390625e-8 * X;   // 2^-24 |x - org|, doubled
-        *D.grid = g;
+        *P.grid = g;
     }
 }
 
 // Our cell key (clamped into the fixed grid) and the reference cell.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D, Gate G, int with_ref) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int with_ref, RunMap M) {
     if (!gate_open(G)) return;
-    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+    const int lim = BATCH ? M.nslot : P0.n;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
+    const Params* Pp = slot_run<BATCH>(P0, M, s);
+    if (!Pp) continue;
+    const Params& P = *Pp;
     double4 q = D.posh[s];
     int4 mt = D.meta[s];
     uint32_t key;
@@ -226,13 +234,14 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P, Dev D, Gate G, int wi
         }
         key = (uint32_t)((int64_t)c[0] + (int64_t)P.nc[0] * ((int64_t)c[1] + (int64_t)P.nc[1] * c[2]));
     }
+    key += (uint32_t)P.c0;
     if (with_ref && mt.y != -1) {
         double xs[3] = {q.x, q.y, q.z};
-        const RefGrid g = *D.grid;
+        const RefGrid g = *P.grid;
         bool bad = false;
         int rc[3] = {1, 1, 1};
         for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
-        if (bad) atomicOr(D.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+        if (bad) atomicOr(P.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
                                                                                   : kErrNonFinite);
         D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
     }
@@ -330,9 +339,12 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count,
 
 // Place each particle in its cell. The decrement leaves every count at zero
 // again for the next step.
-__global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D, Gate G) {
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D, Gate G, RunMap M) {
     if (!gate_open(G)) return;
-    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+    const int lim = BATCH ? M.nslot : P.n;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
+        if (BATCH && M.runof[s] < 0) continue;
         uint32_t key = D.key[s];
         int pos = D.start[key] + atomicSub(&D.count[key], 1) - 1;
         D.perm[pos] = s;
@@ -360,10 +372,13 @@ __global__ __launch_bounds__(kBlock) void k_cellsort(const int* __restrict__ sta
 
 // FP32 positions (relative to the grid origin) in cell order, with the slot
 // in w: the candidate loop of the search streams these 16-byte records.
-__global__ __launch_bounds__(kBlock) void k_cellpos(Params P, Dev D, Gate G) {
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_cellpos(Params P0, Dev D, Gate G, RunMap M) {
     if (!gate_open(G)) return;
-    for (int k = blockIdx.x * kBlock + threadIdx.x; k < P.n; k += gridDim.x * kBlock) {
+    const int lim = BATCH ? M.nact : P0.n;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < lim; k += gridDim.x * kBlock) {
         int j = D.perm[k];
+        const Params& P = BATCH ? M.runs[M.runof[j]] : P0;
         double4 q = D.posh[j];
         D.cpos[k] = make_float4((float)(q.x - P.org[0]), (float)(q.y - P.org[1]),
                                 (float)(q.z - P.org[2]), __int_as_float(j));
@@ -1114,11 +1129,15 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
 
 // Leap-frog correct (timeIntegration.f90:178-252) and bounding-box partials
 // of the new positions for the next step's grid.
-template <int DIM, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_correct(Params P, Dev D, int first) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
+template <int DIM, bool MIXED, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_correct(Params P0, Dev D, int first_arg, RunMap M, int kstep) {
+    int lb;
+    const Params& P = block_run<BATCH>(P0, M, blockIdx.x, lb);
+    const bool first = BATCH ? (P.step0 + kstep == 1) : first_arg != 0;
+    const int l = lb * kBlock + threadIdx.x;
+    const int s = P.s0 + l;
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (s < P.n) {
+    if (l < P.n) {
         int4 mt = D.meta[s];
         double4 q = D.posh[s];
         if (mt.w < P.n_real) {
@@ -1175,11 +1194,11 @@ __global__ __launch_bounds__(kBlock) void k_correct(Params P, Dev D, int first) 
                 }
             }
             if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
-                atomicOr(D.err, kErrNonFinite);
+                atomicOr(P.err, kErrNonFinite);
         }
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
     }
-    bbox_block_reduce(mn, mx, D.partial, blockIdx.x);
+    bbox_block_reduce(mn, mx, P.partial, lb);
 }
 
 // Neighbour statistics of the last search (print_statistics, nnps.f90:377-416):
@@ -1465,17 +1484,17 @@ static void launch_sort(sphgpu_ctx* c, Gate G = Gate{nullptr}, int with_ref = 1)
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const int nbs = std::min(nb, kStrideBlocks);
-    hipLaunchKernelGGL(k_keys<DIM>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, with_ref);
+    hipLaunchKernelGGL((k_keys<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, with_ref, RunMap{});
     const int m = (int)P.ncell + 1;
     const int sb = (m + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum, G);
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb, G);
     hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
                        c->D.start, (int)P.n, G);
-    hipLaunchKernelGGL(k_scatter, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G);
+    hipLaunchKernelGGL(k_scatter<false>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, RunMap{});
     hipLaunchKernelGGL(k_cellsort, dim3(std::min((m + kBlock - 1) / kBlock, kStrideBlocks)), dim3(kBlock),
                        0, c->stream, c->D.start, c->D.perm, m, G);
-    hipLaunchKernelGGL(k_cellpos, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G);
+    hipLaunchKernelGGL(k_cellpos<false>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, RunMap{});
 }
 
 static Gate fast_gate(const sphgpu_ctx* c) { return Gate{c->D.vflag}; }
@@ -1486,11 +1505,11 @@ template <int DIM>
 static void launch_fast_head(sphgpu_ctx* c, int mode) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     if (P.mixed)
-        hipLaunchKernelGGL((k_point_fast<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode);
+        hipLaunchKernelGGL((k_point_fast<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode, RunMap{});
     else
-        hipLaunchKernelGGL((k_point_fast<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode);
+        hipLaunchKernelGGL((k_point_fast<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode, RunMap{});
 }
 
 template <int DIM>
@@ -1499,17 +1518,17 @@ static void launch_fast_tail(sphgpu_ctx* c, int count_only) {
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const Gate G = fast_gate(c);
     launch_sort<DIM>(c, G, 0);
-    hipLaunchKernelGGL(k_verlet_build<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only);
+    hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only, RunMap{});
     if (count_only) return;
     if (c->time_force) hipEventRecord(c->ev[2], c->stream);
     if (P.mixed && P.pair_ext)
-        hipLaunchKernelGGL((k_force_fused<DIM, true, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force_fused<DIM, true, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     else if (P.mixed)
-        hipLaunchKernelGGL((k_force_fused<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force_fused<DIM, true, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     else if (P.pair_ext)
-        hipLaunchKernelGGL((k_force_fused<DIM, false, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force_fused<DIM, false, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     else
-        hipLaunchKernelGGL((k_force_fused<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force_fused<DIM, false, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     if (c->time_force) hipEventRecord(c->ev[3], c->stream);
 }
 
@@ -1533,15 +1552,15 @@ static void advance_fast(sphgpu_ctx* c, int nsteps) {
         const int first = c->step == 1;
         if (k + 1 < nsteps) {
             if (P.mixed)
-                hipLaunchKernelGGL((k_correct_predict<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+                hipLaunchKernelGGL((k_correct_predict<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
             else
-                hipLaunchKernelGGL((k_correct_predict<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
-            hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
-            hipLaunchKernelGGL(k_refcell<DIM>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D);
+                hipLaunchKernelGGL((k_correct_predict<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
+            hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+            hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
         } else if (P.mixed) {
-            hipLaunchKernelGGL((k_correct<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+            hipLaunchKernelGGL((k_correct<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
         } else {
-            hipLaunchKernelGGL((k_correct<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first);
+            hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
         }
         c->step++;
     }
@@ -1571,7 +1590,7 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
         launch_fast<DIM>(c, mode);
         return;
     }
-    hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, P, c->D, nb);
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, mode);
     if (!fast) {
@@ -1605,9 +1624,9 @@ static void launch_step(sphgpu_ctx* c) {
     int first = c->step == 1;
     launch_derivs<DIM>(c, first ? 1 : 0);
     if (c->P.mixed)
-        hipLaunchKernelGGL((k_correct<DIM, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
+        hipLaunchKernelGGL((k_correct<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first, RunMap{}, 0);
     else
-        hipLaunchKernelGGL((k_correct<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
+        hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first, RunMap{}, 0);
 }
 
 static int check_err(sphgpu_ctx* c) {
@@ -1671,6 +1690,17 @@ static int alloc_state(sphgpu_ctx* c) {
     }
     HIPCHK(c, hipMemcpyAsync(D.mat, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream));
     c->P.mat = D.mat;
+    // a single run owns the whole arena
+    c->P.s0 = 0;
+    c->P.c0 = 0;
+    c->P.nblk = (int)((c->n + kBlock - 1) / kBlock);
+    c->P.step0 = 1;
+    c->P.grid = D.grid;
+    c->P.vflag = D.vflag;
+    c->P.vany = D.vflag;
+    c->P.err = D.err;
+    c->P.maxcnt = D.maxcnt;
+    c->P.partial = D.partial;
     return 0;
 }
 
@@ -1700,8 +1730,8 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     // count-only passes on the uploaded state, then capacities with headroom
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
-    hipLaunchKernelGGL(k_grid<DIM>, dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, nb);
+    hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, 2 | 4);
     HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1846,7 +1876,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     if (rc) { c->has_state = false; return rc; }
     // the grid partials of the uploaded positions seed the first step
     const int nb = (int)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     HIPCHK(c, hipGetLastError());
     return check_err(c);
 }
@@ -1854,7 +1884,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
 extern "C" int sphgpu_derivatives(sphgpu_ctx* c) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     // the candidate list of the fast path survives between calls (the
     // displacement vote reopens it), so the result does not depend on how
     // the host splits its calls
@@ -1871,7 +1901,7 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
     if (nsteps < 0) return fail(c, SPHGPU_E_ARG, "negative step count");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     // refresh the grid partials; build the candidate list if none is valid
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     if (!c->fused_done) force_rebuild(c);
     if (nsteps > 0) c->fused_done = fluid_fast_path(c) && c->use_verlet;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -2025,13 +2055,13 @@ extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     if (!offsets) return fail(c, SPHGPU_E_ARG, "offsets required");
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D);
+    hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     if (c->dim == 2) {
-        hipLaunchKernelGGL(k_grid<2>, dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, nb);
+        hipLaunchKernelGGL((k_grid<2, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
         launch_sort<2>(c);
         launch_point_build<2>(c, 2);
     } else {
-        hipLaunchKernelGGL(k_grid<3>, dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, nb);
+        hipLaunchKernelGGL((k_grid<3, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
         launch_sort<3>(c);
         launch_point_build<3>(c, 2);
     }
