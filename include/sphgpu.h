@@ -201,6 +201,76 @@ const char* sphgpu_last_error(sphgpu_ctx* ctx);
 
 void sphgpu_destroy(sphgpu_ctx* ctx);
 
+/* ------------------------------------------------------------------ batches
+ * Many independent runs resident on one GPU, advanced together: every kernel
+ * launch covers all runnable runs of the batch through a device run table.
+ * No counterpart in the reference, which runs one case per process
+ * (src/main.f90:16-22); each run follows the same step as sphgpu_advance
+ * (timeIntegration.f90:137-252 with singleStep.f90:58-144), restricted to the
+ * fluid set of the fast path (cubic spline, continuity density, Tait/linear
+ * EOS, artificial viscosity, gravity, fixed boundary particles).
+ *
+ * A run's results do not depend on the other runs in the batch: its grid,
+ * slot order, candidate-list rebuilds and arithmetic are its own.
+ * One batch = one HIP stream = one host thread. */
+typedef struct sphgpu_batch sphgpu_batch;
+
+/* Device bytes an arena of these dimensions takes (for sizing it from free
+ * HBM). verlet_cap <= 0 selects the default candidate capacity. */
+int sphgpu_batch_bytes(int32_t dim, int32_t precision, int64_t slot_capacity,
+                       int64_t cell_capacity, int32_t max_runs, int32_t verlet_cap,
+                       int64_t* bytes);
+
+/* Allocate an arena of slot_capacity particle slots and cell_capacity cells
+ * for up to max_runs resident runs of one dimension and precision. */
+int sphgpu_batch_create(int32_t dim, int32_t precision, int64_t slot_capacity,
+                        int64_t cell_capacity, int32_t max_runs, int32_t verlet_cap,
+                        sphgpu_batch** out);
+
+/* Admit a run (arrays as for sphgpu_upload) that will advance nsteps steps.
+ * win (cells per cutoff, 1..4) and skin_frac (candidate skin / cutoff)
+ * tune the neighbour search; <= 0 keeps the defaults. Returns
+ * SPHGPU_E_CAPACITY, with the batch unchanged, when the arena or the run
+ * table has no room; *run receives the run's handle. */
+int sphgpu_batch_add(sphgpu_batch* b, const sphgpu_config* cfg, int64_t n_real,
+                     int64_t n_dummy, const double* x, const double* v, const double* mass,
+                     const double* rho, const double* p, const double* e, const double* c,
+                     const double* h, const int32_t* type, const int32_t* state,
+                     const int32_t* boundary, int64_t nsteps, int32_t win, double skin_frac,
+                     int32_t* run);
+
+/* Advance every runnable run by up to nsteps steps; a run stops at its own
+ * last step, and a run that fails (non-finite state, capacity) stops alone
+ * with its status recorded. Returns nonzero only for failures of the batch
+ * itself. */
+int sphgpu_batch_advance(sphgpu_batch* b, int32_t nsteps);
+
+/* State of a run: *next_step = number of the step it would run next;
+ * *state = 0 running, 1 finished, or the negative sphgpu_status it failed
+ * with (message: sphgpu_batch_run_error). */
+int sphgpu_batch_status(sphgpu_batch* b, int32_t run, int64_t* next_step, int32_t* state);
+const char* sphgpu_batch_run_error(sphgpu_batch* b, int32_t run);
+
+/* One field of a run, as sphgpu_download. */
+int sphgpu_batch_download(sphgpu_batch* b, int32_t run, int32_t field, double* dst);
+
+/* Release a run's slots, cells and table row. */
+int sphgpu_batch_remove(sphgpu_batch* b, int32_t run);
+
+/* out8: resident runs, runnable runs, slots in use, slot capacity, cells in
+ * use, cell capacity, max runs, candidate capacity per particle. */
+int sphgpu_batch_info(sphgpu_batch* b, int64_t* out8);
+
+/* Wall time (HIP events on the batch stream) of the last sphgpu_batch_advance,
+ * the steps it ran, and the real-particle steps it advanced over all runs. */
+int sphgpu_batch_timing(sphgpu_batch* b, double* ms, int64_t* steps, int64_t* particle_steps);
+
+/* Message of the last failed batch call (b == NULL: of the last failed
+ * sphgpu_batch_create on this thread). */
+const char* sphgpu_batch_last_error(sphgpu_batch* b);
+
+void sphgpu_batch_destroy(sphgpu_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

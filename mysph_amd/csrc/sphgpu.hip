@@ -1380,14 +1380,12 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
     return 0;
 }
 
-static void fill_params(sphgpu_ctx* c) {
-    const sphgpu_config& g = c->cfg;
-    Params& P = c->P;
+static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_t n_real) {
     P.dim = g.dim;
     P.skf = g.skf;
     P.scale_k = (g.skf == 1) ? 2 : 3;   // asymmetricSearch.f90:178-185
-    P.n = (int)c->n;
-    P.n_real = (int)c->n_real;
+    P.n = (int)n;
+    P.n_real = (int)n_real;
     P.nmat = g.n_materials;
     P.summation_density = g.summation_density;
     P.normalize_density = g.normalize_density;
@@ -1411,20 +1409,37 @@ static void fill_params(sphgpu_ctx* c) {
 // The fused fluid force kernel covers the dam-break physics set; anything
 // else (physical viscosity, solids, kernel correction, summation density,
 // artificial stress, Gauss/quintic kernels) takes the general kernels.
-static bool fluid_fast_path(const sphgpu_ctx* c) {
-    const sphgpu_config& g = c->cfg;
-    return g.skf == 1 && !g.viscosity && !c->P.has_solid && !g.kernel_correction &&
-           !g.summation_density && !g.artificial_stress && !c->force_general;
+static bool fast_path_ok(const sphgpu_config& g, const Params& P, bool force_general) {
+    return g.skf == 1 && !g.viscosity && !P.has_solid && !g.kernel_correction &&
+           !g.summation_density && !g.artificial_stress && !force_general;
 }
+
+static bool fluid_fast_path(const sphgpu_ctx* c) { return fast_path_ok(c->cfg, c->P, c->force_general); }
 
 // Our search grid: cells of cutoff/win (slightly enlarged) over the upload
 // bounding box plus a margin; keys outside are clamped to the border cells,
 // which keeps every neighbour within the +-win sweep.
-static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
-    const int dim = c->dim;
+// Candidate-list tuning of a run: cells per cutoff and skin (fraction of the
+// cutoff). Neither changes which pairs interact; both change the speed.
+struct Tuning {
+    int win;
+    double skin_frac;
+};
+
+static Tuning env_tuning() {
+    const char* wenv = getenv("SPHGPU_WIN");
+    const char* senv = getenv("SPHGPU_SKIN");
+    Tuning t;
+    t.win = (wenv && wenv[0] >= '1' && wenv[0] <= '4') ? wenv[0] - '0' : kWinDefault;
+    t.skin_frac = senv ? atof(senv) : 0.15;
+    return t;
+}
+
+static void plan_grid_params(Params& P, int dim, bool verlet, int64_t n, const double* x,
+                             const double* h, Tuning tune) {
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, hmax = 0;
     for (int d = 0; d < dim; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
-    for (int64_t i = 0; i < c->n; ++i) {
+    for (int64_t i = 0; i < n; ++i) {
         for (int d = 0; d < dim; ++d) {
             double v = x[i * dim + d];
             if (v < mn[d]) mn[d] = v;
@@ -1432,16 +1447,13 @@ static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
         }
         hmax = std::max(hmax, h[i]);
     }
-    const char* wenv = getenv("SPHGPU_WIN");
-    c->P.win = (wenv && wenv[0] >= '1' && wenv[0] <= '4') ? wenv[0] - '0' : kWinDefault;
-    const char* senv = getenv("SPHGPU_SKIN");
-    const double skin_frac = senv ? atof(senv) : 0.15;
-    const double cut_max = hmax * c->P.scale_k;
-    c->P.skin = (fluid_fast_path(c) && c->use_verlet) ? skin_frac * cut_max : 0.0;
-    c->P.hmax = hmax;
-    const double half = 0.5 * c->P.skin * (1.0 - 1e-9);
-    c->P.disp_thr2 = half * half;
-    double cs = (cut_max + c->P.skin) / c->P.win * (1.0 + 1e-6);
+    P.win = tune.win;
+    const double cut_max = hmax * P.scale_k;
+    P.skin = verlet ? tune.skin_frac * cut_max : 0.0;
+    P.hmax = hmax;
+    const double half = 0.5 * P.skin * (1.0 - 1e-9);
+    P.disp_thr2 = half * half;
+    double cs = (cut_max + P.skin) / P.win * (1.0 + 1e-6);
     if (!(cs > 0)) cs = 1.0;
     double lo[3], ext[3];
     for (int d = 0; d < dim; ++d) {
@@ -1450,7 +1462,7 @@ static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
         lo[d] = mn[d] - margin;
         ext[d] = span + 2 * margin;
     }
-    const int64_t budget = std::max<int64_t>(8 * c->n, 4096);
+    const int64_t budget = std::max<int64_t>(8 * n, 4096);
     // per-axis limits of the packed cell coordinates (11/11/9 bits in 3D, 16/15
     // in 2D; bit 31 marks a particle outside the grid)
     const int64_t lim[3] = {dim == 3 ? 2047 : 65535, dim == 3 ? 2047 : 32767, 511};
@@ -1468,7 +1480,6 @@ static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
         if (ok && cells <= budget && cells < (1LL << 31) - 2) break;
         cs *= 1.25;
     }
-    Params& P = c->P;
     for (int d = 0; d < 3; ++d) { P.org[d] = 0; P.nc[d] = 1; }
     for (int d = 0; d < dim; ++d) {
         P.org[d] = lo[d];
@@ -1477,6 +1488,10 @@ static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
     P.inv_cs = 1.0 / cs;
     P.cs = cs;
     P.ncell = (int64_t)P.nc[0] * P.nc[1] * P.nc[2];
+}
+
+static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
+    plan_grid_params(c->P, c->dim, fluid_fast_path(c) && c->use_verlet, c->n, x, h, env_tuning());
 }
 
 template <int DIM>
@@ -1763,26 +1778,43 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     return 0;
 }
 
-extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, const double* x,
-                             const double* v, const double* mass, const double* rho,
-                             const double* p, const double* e, const double* cs,
-                             const double* h, const double* shear, const int32_t* type,
-                             const int32_t* state, const int32_t* boundary) {
-    if (!c) return SPHGPU_E_ARG;
-    if (!x || !mass || !rho || !h || !type || n_real < 0 || n_dummy < 0)
-        return fail(c, SPHGPU_E_ARG, "x, mass, rho, h and type are required");
+// One run's particles in slot order with its parameters and grid plan: what
+// an upload copies to the device. Slots are ordered by our linear cell key (x
+// fastest): neighbouring lanes then hold x-neighbouring particles, so the k-th
+// neighbours of a wavefront's lanes sit in consecutive slots and the gathers
+// coalesce.
+struct HostRun {
+    Params P{};
+    int64_t n = 0, n_real = 0, n_dummy = 0;
+    bool fast = false;
+    std::vector<double4> posh, vm;
+    std::vector<double> rho, e, c, p, shear;
+    std::vector<int4> meta;
+    std::vector<float> drho;   // mixed precision: rho - rho0
+};
+
+static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy, const double* x,
+                       const double* v, const double* mass, const double* rho, const double* p,
+                       const double* e, const double* cs, const double* h, const double* shear,
+                       const int32_t* type, const int32_t* state, const int32_t* boundary,
+                       bool use_verlet, bool force_general, Tuning tune, HostRun& hr,
+                       std::string& why) {
+    if (!x || !mass || !rho || !h || !type || n_real < 0 || n_dummy < 0) {
+        why = "x, mass, rho, h and type are required";
+        return SPHGPU_E_ARG;
+    }
     const int64_t n = n_real + n_dummy;
-    if (n > c->capacity) return fail(c, SPHGPU_E_CAPACITY, "more particles than the context capacity");
-    if (n == 0) return fail(c, SPHGPU_E_ARG, "no particles");
-    if (n >= (1LL << 31) - 1) return fail(c, SPHGPU_E_CAPACITY, "particle count exceeds int32 range");
-    const int dim = c->dim;
-    c->n = n;
-    c->n_real = n_real;
-    c->n_dummy = n_dummy;
-    fill_params(c);
-    c->P.has_solid = 0;
+    if (n == 0) { why = "no particles"; return SPHGPU_E_ARG; }
+    if (n >= (1LL << 31) - 1) { why = "particle count exceeds int32 range"; return SPHGPU_E_CAPACITY; }
+    const int dim = cfg.dim;
+    hr.n = n;
+    hr.n_real = n_real;
+    hr.n_dummy = n_dummy;
+    Params& P = hr.P;
+    fill_params_cfg(P, cfg, n, n_real);
+    P.has_solid = 0;
     for (int64_t i = 0; i < n; ++i)
-        if (type[i] > 100 || type[i] < -100) c->P.has_solid = 1;
+        if (type[i] > 100 || type[i] < -100) P.has_solid = 1;
     {
         bool multi = false;
         int first_pos = 0;
@@ -1791,58 +1823,101 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
                 if (first_pos == 0) first_pos = type[i];
                 else if (type[i] != first_pos) multi = true;
             }
-        c->P.pair_ext = c->cfg.external_force && (c->cfg.lj_r0 > 0 || multi);
+        P.pair_ext = cfg.external_force && (cfg.lj_r0 > 0 || multi);
     }
-    if (c->P.mixed && !fluid_fast_path(c))
-        return fail(c, SPHGPU_E_BAD_CONFIG,
-                    "mixed precision covers the fluid set (cubic spline, continuity density, "
-                    "no physical viscosity, solids, kernel correction or artificial stress)");
-    if (c->cfg.kernel_correction)
+    hr.fast = fast_path_ok(cfg, P, force_general);
+    if (P.mixed && !hr.fast) {
+        why = "mixed precision covers the fluid set (cubic spline, continuity density, "
+              "no physical viscosity, solids, kernel correction or artificial stress)";
+        return SPHGPU_E_BAD_CONFIG;
+    }
+    if (cfg.kernel_correction)
         for (int64_t i = 0; i < n_real; ++i)
-            if (type[i] <= 100)
-                return fail(c, SPHGPU_E_BAD_CONFIG,
-                            "kernel gradient correction with non-solid real particles "
-                            "(the reference divides by a zero matrix there)");
+            if (type[i] <= 100) {
+                why = "kernel gradient correction with non-solid real particles "
+                      "(the reference divides by a zero matrix there)";
+                return SPHGPU_E_BAD_CONFIG;
+            }
     for (int64_t i = 0; i < n * dim; ++i)
-        if (!std::isfinite(x[i])) return fail(c, SPHGPU_E_NONFINITE, "non-finite position");
-    plan_grid(c, x, h);
-    int rc = alloc_state(c);
-    if (rc) return rc;
-    rc = alloc_cells(c);
-    if (rc) return rc;
-    // order the slots once by our linear cell key (x fastest): neighbouring
-    // lanes then hold x-neighbouring particles, so the k-th neighbours of a
-    // wavefront's lanes sit in consecutive slots and the gathers coalesce
+        if (!std::isfinite(x[i])) { why = "non-finite position"; return SPHGPU_E_NONFINITE; }
+    plan_grid_params(P, dim, hr.fast && use_verlet, n, x, h, tune);
     std::vector<std::pair<uint64_t, int64_t>> keyed(n);
     for (int64_t i = 0; i < n; ++i) {
         int64_t cc[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) {
-            double t = std::floor((x[i * dim + d] - c->P.org[d]) * c->P.inv_cs);
-            t = std::min(std::max(t, 0.0), (double)(c->P.nc[d] - 1));
+            double t = std::floor((x[i * dim + d] - P.org[d]) * P.inv_cs);
+            t = std::min(std::max(t, 0.0), (double)(P.nc[d] - 1));
             cc[d] = (int64_t)t;
         }
-        keyed[i] = {(uint64_t)(cc[0] + c->P.nc[0] * (cc[1] + (int64_t)c->P.nc[1] * cc[2])), i};
+        keyed[i] = {(uint64_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2])), i};
     }
     std::sort(keyed.begin(), keyed.end());
-    std::vector<double4> posh(n), vm(n), disp(n), zero4(n);
-    std::vector<double> vrho(n), ve(n), vc(n), vp(n);
-    std::vector<int4> meta(n);
-    std::vector<double> vshear((size_t)n * dim * dim, 0.0);
+    hr.posh.resize(n); hr.vm.resize(n); hr.rho.resize(n); hr.e.resize(n); hr.c.resize(n);
+    hr.p.resize(n); hr.meta.resize(n);
+    hr.shear.assign((size_t)n * dim * dim, 0.0);
     for (int64_t s = 0; s < n; ++s) {
         int64_t i = keyed[s].second;
-        posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
+        hr.posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
         double vx = v ? v[i * dim] : 0, vy = v ? v[i * dim + 1] : 0, vz = (v && dim == 3) ? v[i * dim + 2] : 0;
-        vm[s] = make_double4(vx, vy, vz, mass[i]);
+        hr.vm[s] = make_double4(vx, vy, vz, mass[i]);
+        hr.rho[s] = rho[i];
+        hr.e[s] = e ? e[i] : 0.0;
+        hr.c[s] = cs ? cs[i] : 0.0;
+        hr.p[s] = p ? p[i] : 0.0;
+        hr.meta[s] = make_int4(type[i], state ? state[i] : 0, boundary ? boundary[i] : 0, (int)i);
+        if (shear)
+            for (int q = 0; q < dim * dim; ++q) hr.shear[s * dim * dim + q] = shear[i * dim * dim + q];
+    }
+    if (P.mixed) {
+        hr.drho.resize(n);
+        for (int64_t s = 0; s < n; ++s) {
+            const sphgpu_material* m = nullptr;
+            int a = std::abs(hr.meta[s].x);
+            for (int k = 0; k < cfg.n_materials; ++k)
+                if (cfg.mat[k].type == a) m = &cfg.mat[k];
+            double r0 = m ? m->rho0 : 0.0;   // the same reference density eos_mixed uses
+            hr.drho[s] = (float)(hr.rho[s] - r0);
+        }
+    }
+    return 0;
+}
+
+extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, const double* x,
+                             const double* v, const double* mass, const double* rho,
+                             const double* p, const double* e, const double* cs,
+                             const double* h, const double* shear, const int32_t* type,
+                             const int32_t* state, const int32_t* boundary) {
+    if (!c) return SPHGPU_E_ARG;
+    if (n_real + n_dummy > c->capacity)
+        return fail(c, SPHGPU_E_CAPACITY, "more particles than the context capacity");
+    HostRun hr;
+    std::string why;
+    int rc = prepare_run(c->cfg, n_real, n_dummy, x, v, mass, rho, p, e, cs, h, shear, type, state,
+                         boundary, c->use_verlet, c->force_general, env_tuning(), hr, why);
+    if (rc) return fail(c, rc, why);
+    const int64_t n = hr.n;
+    const int dim = c->dim;
+    c->n = n;
+    c->n_real = n_real;
+    c->n_dummy = n_dummy;
+    c->P = hr.P;
+    rc = alloc_state(c);
+    if (rc) return rc;
+    rc = alloc_cells(c);
+    if (rc) return rc;
+    std::vector<double4> disp(n), zero4(n);
+    for (int64_t s = 0; s < n; ++s) {
         disp[s] = make_double4(0, 0, 0, 0);
         zero4[s] = make_double4(0, 0, 0, 0);
-        vrho[s] = rho[i];
-        ve[s] = e ? e[i] : 0.0;
-        vc[s] = cs ? cs[i] : 0.0;
-        vp[s] = p ? p[i] : 0.0;
-        meta[s] = make_int4(type[i], state ? state[i] : 0, boundary ? boundary[i] : 0, (int)i);
-        if (shear)
-            for (int q = 0; q < dim * dim; ++q) vshear[s * dim * dim + q] = shear[i * dim * dim + q];
     }
+    const std::vector<double4>& posh = hr.posh;
+    const std::vector<double4>& vm = hr.vm;
+    const std::vector<double>& vrho = hr.rho;
+    const std::vector<double>& ve = hr.e;
+    const std::vector<double>& vc = hr.c;
+    const std::vector<double>& vp = hr.p;
+    const std::vector<int4>& meta = hr.meta;
+    const std::vector<double>& vshear = hr.shear;
     Dev& D = c->D;
     HIPCHK(c, hipMemcpyAsync(D.posh, posh.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.vm, vm.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
@@ -1855,16 +1930,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     HIPCHK(c, hipMemcpyAsync(D.meta, meta.data(), n * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.shear, vshear.data(), vshear.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (c->P.mixed) {
-        std::vector<float> dr(n);
-        for (int64_t s = 0; s < n; ++s) {
-            const sphgpu_material* m = nullptr;
-            int a = std::abs(meta[s].x);
-            for (int k = 0; k < c->cfg.n_materials; ++k)
-                if (c->cfg.mat[k].type == a) m = &c->cfg.mat[k];
-            double r0 = m ? m->rho0 : 0.0;   // the same reference density eos_mixed uses
-            dr[s] = (float)(vrho[s] - r0);
-        }
-        HIPCHK(c, hipMemcpyAsync(D.drho, dr.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(D.drho, hr.drho.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     HIPCHK(c, hipMemsetAsync(D.err, 0, sizeof(int) * 4, c->stream));
@@ -1936,17 +2002,25 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
     return rc;
 }
 
-extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
-    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
-    if (!dst || field < 0 || field >= SPHGPU_F_NFIELDS) return fail(c, SPHGPU_E_ARG, "bad field");
-    const int64_t n = c->n;
-    const int dim = c->dim;
+// Copy one field of n particles (slots 0..n-1 of the view D) to dst in the
+// caller's particle order (meta.w).
+#define VCHK(call)                                                           \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) {                                              \
+            msg = std::string(#call) + ": " + hipGetErrorString(e_);         \
+            return SPHGPU_E_HIP;                                             \
+        }                                                                    \
+    } while (0)
+
+static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sphgpu_config& cfg,
+                         hipStream_t stream, int32_t field, double* dst, std::string& msg) {
     std::vector<int4> meta(n);
-    HIPCHK(c, hipMemcpyAsync(meta.data(), c->D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    VCHK(hipMemcpyAsync(meta.data(), D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, stream));
     auto get4 = [&](double4* src, int which) -> int {  // which: 0 xyz, 1 w
         std::vector<double4> h(n);
-        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(double4), hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) {
             int64_t i = meta[s].w;
             if (which == 0) {
@@ -1961,8 +2035,8 @@ extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
     };
     auto get1 = [&](double* src) -> int {
         std::vector<double> h(n);
-        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = h[s];
         return 0;
     };
@@ -1973,16 +2047,16 @@ extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
             return 0;
         }
         std::vector<double> h(n * nn);
-        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        VCHK(hipMemcpyAsync(h.data(), src, n * nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s)
             for (int q = 0; q < nn; ++q) dst[meta[s].w * nn + q] = h[s * nn + q];
         return 0;
     };
     auto getf4 = [&](float4* src, int which) -> int {  // which: 0 xyz, 1 w
         std::vector<float4> h(n);
-        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) {
             int64_t i = meta[s].w;
             if (which == 0) {
@@ -1997,57 +2071,67 @@ extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
     };
     auto getf1 = [&](float* src, bool add_rho0) -> int {
         std::vector<float> h(n);
-        HIPCHK(c, hipMemcpyAsync(h.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) {
             double r0 = 0.0;
             if (add_rho0) {
                 int a = std::abs(meta[s].x);
-                for (int k = 0; k < c->cfg.n_materials; ++k)
-                    if (c->cfg.mat[k].type == a) r0 = c->cfg.mat[k].rho0;
+                for (int k = 0; k < cfg.n_materials; ++k)
+                    if (cfg.mat[k].type == a) r0 = cfg.mat[k].rho0;
             }
             dst[meta[s].w] = r0 + (double)h[s];
         }
         return 0;
     };
-    if (c->P.mixed) {
+    if (mixed) {
         switch (field) {
-        case SPHGPU_F_DVDT: return getf4(c->D.dvf, 0);
-        case SPHGPU_F_DRHODT: return getf4(c->D.dvf, 1);
-        case SPHGPU_F_RHO: return getf1(c->D.drho, true);
-        case SPHGPU_F_DEDT: return getf1(c->D.def, false);
+        case SPHGPU_F_DVDT: return getf4(D.dvf, 0);
+        case SPHGPU_F_DRHODT: return getf4(D.dvf, 1);
+        case SPHGPU_F_RHO: return getf1(D.drho, true);
+        case SPHGPU_F_DEDT: return getf1(D.def, false);
         default: break;
         }
     }
     auto get2 = [&](int which) -> int {
         std::vector<double2> h(n);
-        HIPCHK(c, hipMemcpyAsync(h.data(), c->D.div, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        VCHK(hipMemcpyAsync(h.data(), D.div, n * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = which ? h[s].y : h[s].x;
         return 0;
     };
     switch (field) {
-    case SPHGPU_F_X: return get4(c->D.posh, 0);
-    case SPHGPU_F_H: return get4(c->D.posh, 1);
-    case SPHGPU_F_V: return get4(c->D.vm, 0);
-    case SPHGPU_F_MASS: return get4(c->D.vm, 1);
-    case SPHGPU_F_DISP: return get4(c->D.disp, 0);
-    case SPHGPU_F_DVDT: return get4(c->D.dv, 0);
-    case SPHGPU_F_DRHODT: return get4(c->D.dv, 1);
-    case SPHGPU_F_RHO: return get1(c->D.rho);
-    case SPHGPU_F_P: return get1(c->D.p);
-    case SPHGPU_F_E: return get1(c->D.e);
-    case SPHGPU_F_C: return get1(c->D.c);
-    case SPHGPU_F_DEDT: return get1(c->D.de);
-    case SPHGPU_F_ETA: return get1(c->D.eta);
-    case SPHGPU_F_STRESS: return getT(c->D.stress);
-    case SPHGPU_F_SHEAR: return getT(c->D.shear);
-    case SPHGPU_F_DSDT: return getT(c->D.dsdt);
+    case SPHGPU_F_X: return get4(D.posh, 0);
+    case SPHGPU_F_H: return get4(D.posh, 1);
+    case SPHGPU_F_V: return get4(D.vm, 0);
+    case SPHGPU_F_MASS: return get4(D.vm, 1);
+    case SPHGPU_F_DISP: return get4(D.disp, 0);
+    case SPHGPU_F_DVDT: return get4(D.dv, 0);
+    case SPHGPU_F_DRHODT: return get4(D.dv, 1);
+    case SPHGPU_F_RHO: return get1(D.rho);
+    case SPHGPU_F_P: return get1(D.p);
+    case SPHGPU_F_E: return get1(D.e);
+    case SPHGPU_F_C: return get1(D.c);
+    case SPHGPU_F_DEDT: return get1(D.de);
+    case SPHGPU_F_ETA: return get1(D.eta);
+    case SPHGPU_F_STRESS: return getT(D.stress);
+    case SPHGPU_F_SHEAR: return getT(D.shear);
+    case SPHGPU_F_DSDT: return getT(D.dsdt);
     case SPHGPU_F_DIVX: return get2(0);
     case SPHGPU_F_DIVV: return get2(1);
     default: break;
     }
-    return fail(c, SPHGPU_E_ARG, std::string("field not available: ") + kFieldNames[field]);
+    msg = std::string("field not available: ") + kFieldNames[field];
+    return SPHGPU_E_ARG;
+}
+
+
+extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
+    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
+    if (!dst || field < 0 || field >= SPHGPU_F_NFIELDS) return fail(c, SPHGPU_E_ARG, "bad field");
+    std::string msg;
+    int rc = download_view(c->D, c->n, c->dim, c->P.mixed != 0, c->cfg, c->stream, field, dst, msg);
+    return rc ? fail(c, rc, msg) : 0;
 }
 
 extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr, int64_t cap,
@@ -2157,3 +2241,5 @@ extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
+
+#include "batch.hpp"

@@ -39,7 +39,13 @@ EXPORTED = (
     "sphgpu_advance", "sphgpu_download", "sphgpu_export_pairs", "sphgpu_grid_info",
     "sphgpu_timing", "sphgpu_step", "sphgpu_last_error", "sphgpu_destroy",
     "sphgpu_neighbour_stats",
+    "sphgpu_batch_bytes", "sphgpu_batch_create", "sphgpu_batch_add", "sphgpu_batch_advance",
+    "sphgpu_batch_status", "sphgpu_batch_run_error", "sphgpu_batch_download", "sphgpu_batch_remove",
+    "sphgpu_batch_info", "sphgpu_batch_timing", "sphgpu_batch_last_error", "sphgpu_batch_destroy",
 )
+
+# sphgpu_batch_status states (negative values are the failure's sphgpu_status)
+RUNNING, FINISHED = 0, 1
 
 
 class SphGpuError(RuntimeError):
@@ -77,6 +83,26 @@ def load_library(path: str = LIB_PATH):
     L.sphgpu_last_error.restype = ctypes.c_char_p
     L.sphgpu_destroy.argtypes = [vp]
     L.sphgpu_destroy.restype = None
+    p32 = ctypes.POINTER(ctypes.c_int32)
+    p64 = ctypes.POINTER(i64)
+    L.sphgpu_batch_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32, i64, i64, ctypes.c_int32,
+                                     ctypes.c_int32, p64]
+    L.sphgpu_batch_create.argtypes = [ctypes.c_int32, ctypes.c_int32, i64, i64, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.POINTER(vp)]
+    L.sphgpu_batch_add.argtypes = [vp, ctypes.POINTER(C.Config), i64, i64, dp, dp, dp, dp, dp, dp,
+                                   dp, dp, ip, ip, ip, i64, ctypes.c_int32, ctypes.c_double, p32]
+    L.sphgpu_batch_advance.argtypes = [vp, ctypes.c_int32]
+    L.sphgpu_batch_status.argtypes = [vp, ctypes.c_int32, p64, p32]
+    L.sphgpu_batch_run_error.argtypes = [vp, ctypes.c_int32]
+    L.sphgpu_batch_run_error.restype = ctypes.c_char_p
+    L.sphgpu_batch_download.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, dp]
+    L.sphgpu_batch_remove.argtypes = [vp, ctypes.c_int32]
+    L.sphgpu_batch_info.argtypes = [vp, p64]
+    L.sphgpu_batch_timing.argtypes = [vp, dp, p64, p64]
+    L.sphgpu_batch_last_error.argtypes = [vp]
+    L.sphgpu_batch_last_error.restype = ctypes.c_char_p
+    L.sphgpu_batch_destroy.argtypes = [vp]
+    L.sphgpu_batch_destroy.restype = None
     if L.sphgpu_abi_version() != 1:
         raise RuntimeError("libsphgpu ABI version mismatch")
     _lib = L
@@ -198,6 +224,134 @@ class Simulation:
         if getattr(self, "_ctx", None) and self._ctx.value:
             self._L.sphgpu_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def case_arrays(case: dict) -> dict:
+    """Contiguous host arrays of a case, in the layout the C ABI takes."""
+    n = int(case["n_real"]) + int(case["n_dummy"])
+    return dict(
+        x=_f64(case["x"]), v=_f64(case.get("v")), mass=_f64(case["mass"]),
+        rho=_f64(case["rho"]), p=_f64(case.get("p")), e=_f64(case.get("e")),
+        c=_f64(case.get("c")), h=_f64(case["h"]),
+        type=np.ascontiguousarray(case["type"], dtype=np.int32),
+        state=np.ascontiguousarray(case.get("state", np.zeros(n)), dtype=np.int32),
+        boundary=np.ascontiguousarray(case.get("boundary", np.zeros(n)), dtype=np.int32),
+    )
+
+
+def batch_bytes(dim: int, precision: int, slots: int, cells: int, max_runs: int,
+                verlet_cap: int = 0) -> int:
+    L = load_library()
+    out = ctypes.c_int64(0)
+    rc = L.sphgpu_batch_bytes(dim, precision, slots, cells, max_runs, verlet_cap, ctypes.byref(out))
+    if rc != 0:
+        raise SphGpuError(rc, "bad batch dimensions")
+    return out.value
+
+
+class Batch:
+    """Many runs resident in one arena on the current GPU, advanced together
+    (sphgpu_batch_*). Runs are admitted with add() (False when the arena has
+    no room), advanced with advance(), read with get() and released with
+    remove()."""
+
+    def __init__(self, dim: int, precision: int, slots: int, cells: int, max_runs: int = 256,
+                 verlet_cap: int = 0):
+        L = load_library()
+        self._L = L
+        self.dim = int(dim)
+        self.precision = int(precision)
+        self._b = ctypes.c_void_p()
+        rc = L.sphgpu_batch_create(self.dim, self.precision, int(slots), int(cells), int(max_runs),
+                                   int(verlet_cap), ctypes.byref(self._b))
+        if rc != 0:
+            msg = L.sphgpu_batch_last_error(None)
+            raise SphGpuError(rc, msg.decode() if msg else "sphgpu_batch_create failed")
+        self._n = {}
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self._L.sphgpu_batch_last_error(self._b)
+            raise SphGpuError(rc, msg.decode() if msg else "")
+
+    def add(self, case: dict, nsteps: int, cfg: C.Config | None = None, win: int = 0,
+            skin_frac: float = 0.0):
+        """Admit a run; returns its handle, or None if the batch has no room."""
+        cfg = (cfg if cfg is not None else case["config"]).copy()
+        k = case_arrays(case)
+        n_real, n_dummy = int(case["n_real"]), int(case["n_dummy"])
+        run = ctypes.c_int32(-1)
+        rc = self._L.sphgpu_batch_add(
+            self._b, ctypes.byref(cfg), n_real, n_dummy, _dp(k["x"]), _dp(k["v"]), _dp(k["mass"]),
+            _dp(k["rho"]), _dp(k["p"]), _dp(k["e"]), _dp(k["c"]), _dp(k["h"]), _ip(k["type"]),
+            _ip(k["state"]), _ip(k["boundary"]), int(nsteps), int(win), float(skin_frac),
+            ctypes.byref(run))
+        if rc == -7:   # SPHGPU_E_CAPACITY
+            return None
+        self._check(rc)
+        self._n[run.value] = n_real + n_dummy
+        return run.value
+
+    def advance(self, nsteps: int):
+        self._check(self._L.sphgpu_batch_advance(self._b, int(nsteps)))
+
+    def status(self, run: int):
+        """(next step, state): state 0 running, 1 finished, < 0 failed."""
+        step, state = ctypes.c_int64(), ctypes.c_int32()
+        self._check(self._L.sphgpu_batch_status(self._b, int(run), ctypes.byref(step),
+                                                ctypes.byref(state)))
+        return step.value, state.value
+
+    def run_error(self, run: int) -> str:
+        m = self._L.sphgpu_batch_run_error(self._b, int(run))
+        return m.decode() if m else ""
+
+    def get(self, run: int, name: str) -> np.ndarray:
+        f = C.FIELD_NAMES[name]
+        comp = C.field_components(f, self.dim)
+        n = self._n[run]
+        out = np.zeros(n * comp)
+        self._check(self._L.sphgpu_batch_download(self._b, int(run), f, _dp(out)))
+        if comp == 1:
+            return out
+        if comp == self.dim:
+            return out.reshape(n, self.dim)
+        return out.reshape(n, self.dim, self.dim)
+
+    def remove(self, run: int):
+        self._check(self._L.sphgpu_batch_remove(self._b, int(run)))
+        self._n.pop(run, None)
+
+    def info(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        self._check(self._L.sphgpu_batch_info(self._b, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        keys = ("runs", "runnable", "slots_used", "slots", "cells_used", "cells", "max_runs",
+                "verlet_cap")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def timing(self):
+        """(ms, steps, particle_steps) of the last advance."""
+        ms, st, ps = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.sphgpu_batch_timing(self._b, ctypes.byref(ms), ctypes.byref(st),
+                                                ctypes.byref(ps)))
+        return ms.value, st.value, ps.value
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            self._L.sphgpu_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
 
     def __del__(self):
         try:
