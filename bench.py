@@ -119,6 +119,76 @@ def cpu_baseline(case, seconds_target=20.0):
                        f"{threads} OpenMP threads")
 
 
+def serving_leg(args):
+    """Continuous batching of the dam-break parameter sweep: tune the
+    per-size tiling, find the saturation rate with all requests arriving at
+    once, then serve Poisson streams at fractions of it."""
+    import math
+    from mysph_amd import config as C
+    from mysph_amd import serving
+    from mysph_amd.sphgpu import Batch
+
+    t0 = time.perf_counter()
+    reqs = serving.sweep_requests(args.serving_runs, steps=args.serving_steps)
+    setup_s = time.perf_counter() - t0
+    work = sum(r.n_fluid * r.steps for r in reqs)
+    log(f"serving: {len(reqs)} requests, {sum(r.n_fluid for r in reqs)} fluid particles, "
+        f"case set-up {setup_s:.1f} s")
+
+    def make_batch(slots=None):
+        return Batch(2, C.MIXED, slots=slots or args.serving_slots, cells=8 * (slots or args.serving_slots),
+                     max_runs=256)
+
+    tuner = serving.Tuner()
+    reps = {0: 10000, 1: 40000, 2: 150000, 3: 450000}
+    report = tuner.tune(make_batch, lambda k: serving.dam_break_request_case(reps[k], 1.25, 0.1),
+                        fill=2_000_000, warmup=5, steps=20)
+    log(f"serving: tuned tiling per size class {tuner.table}")
+
+    def serve(rate):
+        serving.poisson_arrivals(reqs, rate)
+        with make_batch() as b:
+            srv = serving.Server(b, chunk=args.serving_chunk, tuner=tuner)
+            res = srv.serve(reqs)
+        st = serving.latency_stats(res)
+        st["advance_s"] = srv.advance_s
+        st["chunks"] = srv.chunks
+        return st
+
+    burst = serve(math.inf)
+    mean_work = work / len(reqs)
+    sat_rate = burst["throughput"] / mean_work       # requests/s the GPU sustains
+    log(f"serving: burst p50 {burst['p50_s']:.2f} s p99 {burst['p99_s']:.2f} s, "
+        f"{burst['throughput']:.4g} particle-steps/s, saturation {sat_rate:.3g} runs/s")
+    streams = []
+    for f in args.serving_rates:
+        st = serve(f * sat_rate)
+        st.update(rate_factor=f, rate_runs_per_s=f * sat_rate)
+        streams.append(st)
+        log(f"serving: {f:g}x saturation: p50 {st['p50_s']:.2f} s p99 {st['p99_s']:.2f} s, "
+            f"{st['throughput']:.4g} particle-steps/s")
+    head = streams[0] if streams else burst
+    return {
+        "workload": f"{len(reqs)} 2D dam-break runs, fluid counts log-uniform in [5k, 500k] "
+                    f"(seed {serving.SEED_SWEEP}), column aspect U[0.5,2], alpha U[0.05,0.2], "
+                    f"{args.serving_steps} steps each, mixed precision, Poisson arrivals",
+        "runs": len(reqs),
+        "steps_per_run": args.serving_steps,
+        "particle_steps_total": int(work),
+        "arena_slots": args.serving_slots,
+        "chunk_steps": args.serving_chunk,
+        "tiling_per_size_class": {"bounds": list(tuner.bounds), "win_skin": tuner.table,
+                                  "ns_per_particle_step": {str(k): v for k, v in report.items()}},
+        "headline_rate_factor": head.get("rate_factor", "burst"),
+        "p50_s": head["p50_s"],
+        "p99_s": head["p99_s"],
+        "throughput": head["throughput"],
+        "burst": burst,
+        "saturation_runs_per_s": sat_rate,
+        "poisson": streams,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,9 +196,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.0075)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64", "serving"], default="both",
+                    help="serving: only the batched serving leg")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed run of both precisions to report the mixed density error")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched serving leg")
+    ap.add_argument("--serving-runs", type=int, default=256)
+    ap.add_argument("--serving-steps", type=int, default=2000)
+    ap.add_argument("--serving-slots", type=int, default=8_000_000,
+                    help="arena size in particle slots (admission waits for room)")
+    ap.add_argument("--serving-chunk", type=int, default=50)
+    ap.add_argument("--serving-rates", type=float, nargs="*", default=[0.5, 1.0],
+                    help="Poisson arrival rates as fractions of the measured saturation rate")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -147,6 +226,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "serving":
+        print(json.dumps({"serving": serving_leg(args)}), flush=True)
+        return
     from mysph_amd import cases
     from mysph_amd import config as C
     case = cases.dam_break_3d(dx=args.dx)
@@ -215,6 +297,8 @@ def main():
         out["mixed_rho_rel_err_vs_fp64"] = float(d)
     if "fp64" in res and "mixed" in res and args.accuracy_steps > 0:
         out["accuracy"] = accuracy(case, args.accuracy_steps)
+    if not args.no_serving:
+        out["serving"] = serving_leg(args)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(case)

@@ -428,7 +428,9 @@ static int rebuild_map(sphgpu_batch* b) {
 
 template <int DIM, bool MIXED>
 static void batch_chunk(sphgpu_batch* b, int m) {
-    const RunMap M{b->d_runs, b->d_blk, b->d_runof, b->d_rows, (int)b->slots.high_water(), b->nact};
+    const int mc = (int)b->cells.high_water();
+    const RunMap M{b->d_runs, b->d_blk, b->d_runof, b->d_rows, (int)b->slots.high_water(), b->nact,
+                   b->D.start + mc};
     const Params P0{};
     Dev& D = b->D;
     hipStream_t st = b->stream;
@@ -436,7 +438,6 @@ static void batch_chunk(sphgpu_batch* b, int m) {
     const int nr = (int)b->rows.size();
     const int nss = std::max(1, std::min((M.nslot + kBlock - 1) / kBlock, kStrideBlocks));
     const int nsa = std::max(1, std::min((M.nact + kBlock - 1) / kBlock, kStrideBlocks));
-    const int mc = (int)b->cells.high_water();
     const int sb = (mc + kScanTile - 1) / kScanTile;
     const Gate G{b->d_vany};
     hipLaunchKernelGGL(k_bbox<true>, dim3(nbt), dim3(kBlock), 0, st, P0, D, M);
@@ -446,7 +447,7 @@ static void batch_chunk(sphgpu_batch* b, int m) {
         hipLaunchKernelGGL((k_keys<DIM, true>), dim3(nss), dim3(kBlock), 0, st, P0, D, G, 0, M);
         hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, G);
         hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, st, D.bsum, sb, G);
-        hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, D.start, M.nact, G);
+        hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, D.start, G);
         hipLaunchKernelGGL(k_scatter<true>, dim3(nss), dim3(kBlock), 0, st, P0, D, G, M);
         hipLaunchKernelGGL(k_cellsort, dim3(std::max(1, std::min((mc + kBlock - 1) / kBlock, kStrideBlocks))),
                            dim3(kBlock), 0, st, D.start, D.perm, mc, G);

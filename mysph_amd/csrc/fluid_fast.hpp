@@ -64,70 +64,79 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
     const Gate G = BATCH ? Gate{P.vflag} : G0;
     const int l = lb * kBlock + threadIdx.x;
     const int s = P.s0 + l;
-    if (l >= P.n || !gate_open(G)) return;
-    const double4 qi = D.posh[s];
-    D.xb[s] = qi;
-    const int4 mt = D.meta[s];
-    if (mt.w >= P.n_real) return;
-    if (mt.y == -1) {
-        D.vcnt[s] = 0;
-        return;
+    if (!gate_open(G)) return;   // uniform over the block (one run, one gate)
+    // every lane stays to the wave reduction at the end; only lanes of
+    // particles touch memory
+    bool active = false;
+    double4 qi = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (l < P.n) {
+        qi = D.posh[s];
+        D.xb[s] = qi;
+        const int4 mt = D.meta[s];
+        active = mt.w < P.n_real && mt.y != -1;
+        if (mt.w < P.n_real && mt.y == -1) D.vcnt[s] = 0;
     }
-    const double cutv = qi.w * (double)P.scale_k + P.skin;
-    const RefGrid g = *P.grid;
-    const double u = 5.9604644775390625e-8;
-    const double e = g.pos_err + u * cutv;
-    const double M = 2.0 * 1.7320508075688772 * cutv * e + 3.0 * e * e + 3.0 * u * cutv * cutv;
-    const float lim = (float)(cutv * cutv + 2.0 * M + 1e-6 * cutv * cutv);
-    const float xf = (float)(qi.x - P.org[0]), yf = (float)(qi.y - P.org[1]), zf = (float)(qi.z - P.org[2]);
-    const uint32_t key = D.key[s] - (uint32_t)P.c0;   // cell within the run's grid
-    const int* __restrict__ start = D.start + P.c0;
-    const int cx = (int)(key % (uint32_t)P.nc[0]);
-    const int rest = (int)(key / (uint32_t)P.nc[0]);
-    const int cy = rest % P.nc[1];
-    const int cz = rest / P.nc[1];
-    const int W = P.win;
-    const int zlo = DIM == 3 ? max(cz - W, 0) : 0, zhi = DIM == 3 ? min(cz + W, P.nc[2] - 1) : 0;
-    const int ylo = max(cy - W, 0), yhi = min(cy + W, P.nc[1] - 1);
-    const int xlo = max(cx - W, 0), xhi = min(cx + W, P.nc[0] - 1);
     const int cap = count_only ? 0 : P.verlet_cap;
     int cnt = 0;
-    auto test = [&](const float4 cj) {
-        const float dx = xf - cj.x, dy = yf - cj.y, dz = DIM == 3 ? zf - cj.z : 0.0f;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const int j = __float_as_int(cj.w);
-        if (r2 < lim && j != s) {
-            if (cnt < cap) D.vnbr[(int64_t)cnt * P.ldn + s] = j;
-            ++cnt;
+    if (active) {
+        const double cutv = qi.w * (double)P.scale_k + P.skin;
+        const RefGrid g = *P.grid;
+        const double u = 5.9604644775390625e-8;
+        const double e = g.pos_err + u * cutv;
+        const double M = 2.0 * 1.7320508075688772 * cutv * e + 3.0 * e * e + 3.0 * u * cutv * cutv;
+        const float lim = (float)(cutv * cutv + 2.0 * M + 1e-6 * cutv * cutv);
+        const float xf = (float)(qi.x - P.org[0]), yf = (float)(qi.y - P.org[1]), zf = (float)(qi.z - P.org[2]);
+        const uint32_t key = D.key[s] - (uint32_t)P.c0;   // cell within the run's grid
+        const int* __restrict__ start = D.start + P.c0;
+        const int cx = (int)(key % (uint32_t)P.nc[0]);
+        const int rest = (int)(key / (uint32_t)P.nc[0]);
+        const int cy = rest % P.nc[1];
+        const int cz = rest / P.nc[1];
+        const int W = P.win;
+        const int zlo = DIM == 3 ? max(cz - W, 0) : 0, zhi = DIM == 3 ? min(cz + W, P.nc[2] - 1) : 0;
+        const int ylo = max(cy - W, 0), yhi = min(cy + W, P.nc[1] - 1);
+        const int xlo = max(cx - W, 0), xhi = min(cx + W, P.nc[0] - 1);
+        auto test = [&](const float4 cj) {
+            const float dx = xf - cj.x, dy = yf - cj.y, dz = DIM == 3 ? zf - cj.z : 0.0f;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const int j = __float_as_int(cj.w);
+            if (r2 < lim && j != s) {
+                if (cnt < cap) D.vnbr[(int64_t)cnt * P.ldn + s] = j;
+                ++cnt;
+            }
+        };
+        int z = zlo, y = ylo;
+        int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
+        int kb = start[row + xlo], ke = start[row + xhi + 1];
+        for (;;) {
+            int ny = y + 1, nz = z;
+            if (ny > yhi) { ny = ylo; ++nz; }
+            const bool more = nz <= zhi;
+            int nkb = 0, nke = 0;
+            if (more) {
+                const int64_t nrow = (int64_t)P.nc[0] * ((int64_t)ny + (int64_t)P.nc[1] * nz);
+                nkb = start[nrow + xlo];
+                nke = start[nrow + xhi + 1];
+            }
+            int k = kb;
+            for (; k + 1 < ke; k += 2) {
+                const float4 c0 = D.cpos[k];
+                const float4 c1 = D.cpos[k + 1];
+                test(c0);
+                test(c1);
+            }
+            if (k < ke) test(D.cpos[k]);
+            if (!more) break;
+            y = ny; z = nz; kb = nkb; ke = nke;
         }
-    };
-    int z = zlo, y = ylo;
-    int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
-    int kb = start[row + xlo], ke = start[row + xhi + 1];
-    for (;;) {
-        int ny = y + 1, nz = z;
-        if (ny > yhi) { ny = ylo; ++nz; }
-        const bool more = nz <= zhi;
-        int nkb = 0, nke = 0;
-        if (more) {
-            const int64_t nrow = (int64_t)P.nc[0] * ((int64_t)ny + (int64_t)P.nc[1] * nz);
-            nkb = start[nrow + xlo];
-            nke = start[nrow + xhi + 1];
-        }
-        int k = kb;
-        for (; k + 1 < ke; k += 2) {
-            const float4 c0 = D.cpos[k];
-            const float4 c1 = D.cpos[k + 1];
-            test(c0);
-            test(c1);
-        }
-        if (k < ke) test(D.cpos[k]);
-        if (!more) break;
-        y = ny; z = nz; kb = nkb; ke = nke;
+        D.vcnt[s] = cnt;
+        if (!count_only && cnt > P.verlet_cap) atomicOr(P.err, kErrPairCap);
     }
-    D.vcnt[s] = cnt;
-    if (!count_only && cnt > P.verlet_cap) atomicOr(P.err, kErrPairCap);
-    atomicMax(P.maxcnt, cnt);
+    // one atomic per wave for the largest count (same-address atomics from
+    // every lane serialise in L2)
+    int wmax = cnt;
+    for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+    if ((threadIdx.x & 63) == 0 && wmax > __atomic_load_n(P.maxcnt, __ATOMIC_RELAXED)) atomicMax(P.maxcnt, wmax);
 }
 
 // Exact neighbour test of one candidate in FP64, the reference's arithmetic:

@@ -99,7 +99,8 @@ struct RunMap {
     const int* runof;     // slot -> run row; -1 marks a free slot
     const int* rows;      // rows of the active runs (one block each in k_grid)
     int nslot;            // slots below the arena's high-water mark
-    int nact;             // particles of the active runs (entries of the cell order)
+    int nact;             // particles of the runnable runs
+    const int* nsorted;   // entries of the last cell sort (written by the scan)
 };
 
 template <bool BATCH>

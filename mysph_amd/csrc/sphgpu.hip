@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int w
     const int lim = BATCH ? M.nslot : P0.n;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
     const Params* Pp = slot_run<BATCH>(P0, M, s);
-    if (!Pp) continue;
+    if (!Pp || (BATCH && !gate_open(Gate{Pp->vflag}))) continue;   // only runs that rebuild
     const Params& P = *Pp;
     double4 q = D.posh[s];
     int4 mt = D.meta[s];
@@ -303,12 +303,13 @@ __global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb, Gate G) {
         carry += tot;
         __syncthreads();
     }
+    if (threadIdx.x == 0) bsum[nb] = carry;   // entries sorted in all
 }
 
 // Cell start table: start[c] = exclusive prefix of count; the tile goes through
 // LDS so that global loads and stores stay coalesced.
 __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count, int m,
-                                                  const int* bsum, int* start, int total, Gate G) {
+                                                  const int* bsum, int* start, Gate G) {
     if (!gate_open(G)) return;
     __shared__ int tile[kScanTile];
     const int base = blockIdx.x * kScanTile;
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count,
         int i = threadIdx.x + k * kBlock;
         if (base + i < m) start[base + i] = tile[i];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) start[m] = total;
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[m] = bsum[gridDim.x];
 }
 
 // Place each particle in its cell. The decrement leaves every count at zero
@@ -344,7 +345,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D, Gate G, Run
     if (!gate_open(G)) return;
     const int lim = BATCH ? M.nslot : P.n;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
-        if (BATCH && M.runof[s] < 0) continue;
+        if (BATCH) {
+            const Params* Pp = slot_run<BATCH>(P, M, s);
+            if (!Pp || !gate_open(Gate{Pp->vflag})) continue;
+        }
         uint32_t key = D.key[s];
         int pos = D.start[key] + atomicSub(&D.count[key], 1) - 1;
         D.perm[pos] = s;
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_cellsort(const int* __restrict__ sta
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_cellpos(Params P0, Dev D, Gate G, RunMap M) {
     if (!gate_open(G)) return;
-    const int lim = BATCH ? M.nact : P0.n;
+    const int lim = BATCH ? *M.nsorted : P0.n;
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < lim; k += gridDim.x * kBlock) {
         int j = D.perm[k];
         const Params& P = BATCH ? M.runs[M.runof[j]] : P0;
@@ -627,7 +631,8 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
     }
     D.ncnt[s] = cnt;
     if (cnt > P.nbr_cap && !(mode & 4)) atomicOr(D.err, kErrPairCap);
-    atomicMax(D.maxcnt, cnt);
+    // same-address atomics serialise: only a count above the stored maximum goes
+    if (cnt > __atomic_load_n(D.maxcnt, __ATOMIC_RELAXED)) atomicMax(D.maxcnt, cnt);
 }
 
 // Pair geometry of target i and neighbour j: dx = x_i - x_j, r, W, grad W.
@@ -1505,7 +1510,7 @@ static void launch_sort(sphgpu_ctx* c, Gate G = Gate{nullptr}, int with_ref = 1)
     hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum, G);
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb, G);
     hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
-                       c->D.start, (int)P.n, G);
+                       c->D.start, G);
     hipLaunchKernelGGL(k_scatter<false>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, RunMap{});
     hipLaunchKernelGGL(k_cellsort, dim3(std::min((m + kBlock - 1) / kBlock, kStrideBlocks)), dim3(kBlock),
                        0, c->stream, c->D.start, c->D.perm, m, G);
