@@ -102,7 +102,7 @@ int64_t batch_bytes(int dim, int precision, int64_t slots, int64_t cells, int ma
     b += N * (sizeof(float4));                         // cpos
     b += N * (int64_t)vcap * sizeof(int);              // candidate lists
     if (precision == SPHGPU_MIXED)
-        b += N * (sizeof(float4) * 4 + sizeof(float) * 3);   // vmf thf posf dvf, drho drho_prev def
+        b += N * (sizeof(float4) * 4 + sizeof(float) * 3);   // posf vmf thf dvf, drho drho_prev def
     else
         b += N * (sizeof(double4) * 2 + sizeof(double));     // dv th, de
     b += (N / kBlock + 1) * 6 * sizeof(double) + (N / kBlock + 1) * sizeof(int2);
@@ -573,7 +573,7 @@ extern "C" int sphgpu_batch_download(sphgpu_batch* b, int32_t run, int32_t field
     const int64_t s0 = r->s0;
     auto off = [&](auto*& p) { if (p) p += s0; };
     off(V.posh); off(V.vm); off(V.rho); off(V.e); off(V.c); off(V.p); off(V.meta); off(V.disp);
-    off(V.dv); off(V.de); off(V.div); off(V.eta); off(V.vmf); off(V.thf); off(V.drho); off(V.dvf);
+    off(V.dv); off(V.de); off(V.div); off(V.eta); off(V.drho); off(V.dvf);
     off(V.def);
     std::string msg;
     int rc = download_view(V, r->n, b->dim, b->precision == SPHGPU_MIXED, r->cfg, b->stream, field, dst, msg);

@@ -268,36 +268,56 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
         }
         ++acc_n;
         R4 vj, thj;
-        R hj, Vj;
-        if (MIXED) {
+        R hj, Vj, mj, r, G;
+        if constexpr (MIXED) {
+            // vmf.w = m / rho; one rsq gives r and 1 / r; both kernel
+            // branches are cheap, so select instead of diverging
             vj = *(const R4*)&D.vmf[j];
             thj = *(const R4*)&D.thf[j];
             hj = thj.w;
-            Vj = vj.w * fast_rcp(thj.x);
+            Vj = vj.w;
+            mj = Vj * thj.x;
+            const R rinv = fast_rsq(r2);
+            r = r2 * rinv;
+            const R q = r * inv_h;
+            const R t = (R)2 - q;
+            const R Gn = g1 * ((R)-2 + (R)1.5 * q);
+            const R Gf = g2 * t * t * rinv;
+            G = q < (R)1 ? Gn : Gf;
         } else {
             vj = *(const R4*)&D.vm[j];
             thj = *(const R4*)&D.th[j];
             hj = (R)D.posh[j].w;
             Vj = thj.w;
+            mj = vj.w;
+            r = fast_sqrt(r2);
+            const R q = r * inv_h;
+            const R t = (R)2 - q;
+            G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
         }
-        const R r = fast_sqrt(r2);
-        const R q = r * inv_h;
-        const R t = (R)2 - q;
-        const R G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
         const R gx = G * dx, gy = G * dy, gz = G * dz;
         const R dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = DIM == 3 ? vi.z - vj.z : (R)0;
         const R dvg = dvx * gx + dvy * gy + dvz * gz;
         const R dxg = dx * gx + dy * gy + dz * gz;
-        const R mj = vj.w;
         sv += Vj * dvg;
         sx += Vj * dxg;
         const R aux = thi.y + thj.y;
         const R xv = dx * dvx + dy * dvy + dz * dvz;
         const R hij = (R)0.5 * (hi + hj);
-        const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
-        const R Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
-                                     fast_rcp((R)0.5 * (thi.x + thj.x))
-                               : (R)0;
+        R Pi;
+        if constexpr (MIXED) {
+            // evaluated for every pair and selected: a branch here lets the
+            // compiler split the neighbour record load and wait twice
+            const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
+            const R piv = (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
+                          fast_rcp((R)0.5 * (thi.x + thj.x));
+            Pi = xv < (R)0 ? piv : (R)0;
+        } else {
+            const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
+            Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
+                                 fast_rcp((R)0.5 * (thi.x + thj.x))
+                           : (R)0;
+        }
         const R cf = mj * (aux + Pi);
         ax -= cf * gx;
         ay -= cf * gy;
@@ -375,16 +395,20 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
             const double4 vp = D.vprev[s];
             double dvx, dvy, dvz, dee;
             bool finite_rho;
-            double4 v = D.vm[s];
+            // mixed: vprev.w holds the mass and the FP64 predicted velocity is
+            // not stored (the pair pass reads its FP32 copy; the closing
+            // correct writes the FP64 state)
+            double4 v = MIXED ? make_double4(0.0, 0.0, 0.0, vp.w) : D.vm[s];
             double e, rho_c = 0.0;
+            float dr_pred = 0.0f;
             if (MIXED) {
                 const float4 dv = D.dvf[s];
                 dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
                 float dr = D.drho_prev[s] + (float)fac * dv.w;   // correct
                 finite_rho = isfinite(dr);
                 D.drho_prev[s] = dr;
-                dr = dr + (float)h2 * dv.w;                      // predict of the next step
-                D.drho[s] = dr;
+                dr_pred = dr + (float)h2 * dv.w;                 // predict of the next step
+                D.drho[s] = dr_pred;
             } else {
                 const double4 dv = D.dv[s];
                 dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
@@ -414,14 +438,14 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
             if (DIM == 3) q.z = q.z + dt * v.z;
             D.posh[s] = q;
             // predict of the next step: prev = corrected state, then half kick
-            D.vprev[s] = make_double4(v.x, v.y, v.z, rho_c);
+            D.vprev[s] = make_double4(v.x, v.y, v.z, MIXED ? vp.w : rho_c);
             v.x = v.x + h2 * dvx;
             v.y = v.y + h2 * dvy;
             if (DIM == 3) v.z = v.z + h2 * dvz;
-            D.vm[s] = v;
+            if (!MIXED) D.vm[s] = v;
             if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
                 atomicOr(P.err, kErrNonFinite);
-            if (MIXED) eos_mixed(P, D, s, mt.x);
+            if (MIXED) eos_mixed_vals(P, D, s, mt.x, dr_pred, v, q);
             else eos_th(P, D, s, mt);
             const double4 b = D.xb[s];
             const double ex = q.x - b.x, ey = q.y - b.y, ez = q.z - b.z;

@@ -81,8 +81,11 @@ struct Dev {
     // mixed precision: FP32 copies read by the pair kernels, and FP32 density
     // deviation / derivatives (positions and velocities stay FP64)
     float4* cpos;     // FP32 positions relative to the grid origin, in cell order; w = slot
-    float4* posf;     // mixed: offset from own cell corner (xyz) + packed cell (w)
-    float4* vmf;      // vx, vy, vz, mass
+    // mixed: separate 16-byte records. The candidate test reads only posf, and
+    // the k-th candidates of neighbouring lanes are neighbouring slots, so its
+    // gathers coalesce best when posf is packed on its own
+    float4* posf;     // offset from own cell corner (xyz) + packed cell (w)
+    float4* vmf;      // vx, vy, vz, m / rho
     float4* thf;      // rho, p/rho^2, c, h
     float* drho;      // rho - rho0
     float* drho_prev;
@@ -445,7 +448,7 @@ __device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int
     double4 v = D.vm[s];
     float dr = D.drho[s];
     double e = D.e[s];
-    D.vprev[s] = make_double4(v.x, v.y, v.z, 0.0);
+    D.vprev[s] = make_double4(v.x, v.y, v.z, v.w);   // mixed: w holds the mass
     D.drho_prev[s] = dr;
     D.eprev[s] = e;
     if (first) return;
@@ -465,9 +468,9 @@ __device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int
 // FP32 EOS on the density deviation: the Tait term (rho/rho0)^gamma - 1 is
 // evaluated as a polynomial in u = drho/rho0 (gamma = 7) or expm1(gamma
 // log1p(u)), so it does not cancel.
-__device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, int type) {
+__device__ __forceinline__ void eos_mixed_vals(const Params& P, const Dev& D, int s, int type, float dr,
+                                               const double4& v, const double4& q) {
     const Material* m = find_mat(P, type);
-    float dr = D.drho[s];
     float rho0 = m ? (float)m->rho0 : 0.0f;
     float rho = rho0 + dr;
     float p;
@@ -486,12 +489,12 @@ __device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, 
         p = (float)D.p[s];
     }
     D.p[s] = (double)p;
-    double4 v = D.vm[s];
     float mass = (float)v.w;
     float irho = 1.0f / rho;
-    double4 q = D.posh[s];
     D.thf[s] = make_float4(rho, p * irho * irho, (float)D.c[s], (float)q.w);
-    D.vmf[s] = make_float4((float)v.x, (float)v.y, (float)v.z, mass);
+    // w = m / rho: the pair loop needs the volume of every neighbour and
+    // recovers the mass as V rho (one multiply instead of a reciprocal per pair)
+    D.vmf[s] = make_float4((float)v.x, (float)v.y, (float)v.z, mass * irho);
     // offset from the particle's own cell corner, exact cell size cs: dx of a
     // pair is (off_i - off_j) + (c_i - c_j) cs, accurate to ~1e-7 of a cell
     double xs[3] = {q.x, q.y, q.z};
@@ -512,6 +515,10 @@ __device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, 
                              : ((uint32_t)c[0] | ((uint32_t)c[1] << 16));
     if (escaped) pk |= 0x80000000u;
     D.posf[s] = make_float4(off[0], off[1], off[2], __uint_as_float(pk));
+}
+
+__device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, int type) {
+    eos_mixed_vals(P, D, s, type, D.drho[s], D.vm[s], D.posh[s]);
 }
 
 // Predict + EOS for every particle, then the neighbour search for real
@@ -985,6 +992,7 @@ __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 template <int DIM>
 __device__ __forceinline__ void unpack_cell(uint32_t pk, int& x, int& y, int& z) {
@@ -1047,7 +1055,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
             dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
             dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
             hj = thj.w;
-            Vj = vj.w * fast_rcp(thj.x);
+            Vj = vj.w;              // vmf.w = m / rho
+            vj.w = Vj * thj.x;      // mass
         } else {
             vj = *(const R4*)&D.vm[j];
             thj = *(const R4*)&D.th[j];
