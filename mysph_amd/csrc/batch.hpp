@@ -95,16 +95,16 @@ constexpr int kBatchVerletCap3D = 160;
 int64_t batch_bytes(int dim, int precision, int64_t slots, int64_t cells, int max_runs, int vcap) {
     const int64_t N = slots;
     int64_t b = 0;
-    b += N * (sizeof(double4) * 5);                    // posh vm disp vprev xb
+    b += N * (sizeof(double4) * 3);                    // posh vm vprev
     b += N * (sizeof(double) * 6);                     // rho e c p eprev eta
     b += N * (sizeof(int4) + sizeof(double2));         // meta div
     b += N * (sizeof(uint32_t) * 2 + sizeof(int) * 4); // key refcell perm vcnt ecnt runof
     b += N * (sizeof(float4));                         // cpos
     b += N * (int64_t)vcap * sizeof(int);              // candidate lists
     if (precision == SPHGPU_MIXED)
-        b += N * (sizeof(float4) * 4 + sizeof(float) * 3);   // posf vmf thf dvf, drho drho_prev def
+        b += N * (sizeof(float4) * 6 + sizeof(float) * 3);   // posf vmf thf dvf dispf xbf, drho drho_prev def
     else
-        b += N * (sizeof(double4) * 2 + sizeof(double));     // dv th, de
+        b += N * (sizeof(double4) * 4 + sizeof(double));     // dv th disp xb, de
     b += (N / kBlock + 1) * 6 * sizeof(double) + (N / kBlock + 1) * sizeof(int2);
     b += (cells + 2) * sizeof(int) * 2 + ((cells + kScanTile) / kScanTile + 1) * sizeof(int);
     b += (int64_t)max_runs * (sizeof(Params) + sizeof(RefGrid) + 2 * sizeof(unsigned long long) +
@@ -222,18 +222,20 @@ extern "C" int sphgpu_batch_create(int32_t dim, int32_t precision, int64_t slot_
     int rc = 0;
     rc |= balloc(b, &D.posh, N); rc |= balloc(b, &D.vm, N); rc |= balloc(b, &D.rho, N);
     rc |= balloc(b, &D.e, N); rc |= balloc(b, &D.c, N); rc |= balloc(b, &D.p, N);
-    rc |= balloc(b, &D.meta, N); rc |= balloc(b, &D.disp, N); rc |= balloc(b, &D.vprev, N);
+    rc |= balloc(b, &D.meta, N); rc |= balloc(b, &D.vprev, N);
     rc |= balloc(b, &D.eprev, N); rc |= balloc(b, &D.div, N); rc |= balloc(b, &D.eta, N);
     rc |= balloc(b, &D.key, N); rc |= balloc(b, &D.refcell, N); rc |= balloc(b, &D.perm, N);
-    rc |= balloc(b, &D.cpos, N); rc |= balloc(b, &D.xb, N); rc |= balloc(b, &D.vcnt, N);
+    rc |= balloc(b, &D.cpos, N); rc |= balloc(b, &D.vcnt, N);
     rc |= balloc(b, &D.ecnt, N);
     rc |= balloc(b, &D.vnbr, (size_t)N * b->verlet_cap);
     if (precision == SPHGPU_MIXED) {
+        rc |= balloc(b, &D.dispf, N); rc |= balloc(b, &D.xbf, N);
         rc |= balloc(b, &D.vmf, N); rc |= balloc(b, &D.thf, N); rc |= balloc(b, &D.posf, N);
         rc |= balloc(b, &D.dvf, N); rc |= balloc(b, &D.drho, N); rc |= balloc(b, &D.drho_prev, N);
         rc |= balloc(b, &D.def, N);
     } else {
         rc |= balloc(b, &D.th, N); rc |= balloc(b, &D.de, N); rc |= balloc(b, &D.dv, N);
+        rc |= balloc(b, &D.disp, N); rc |= balloc(b, &D.xb, N);
     }
     rc |= balloc(b, &D.partial, (size_t)(N / kBlock + 1) * 6);
     rc |= balloc(b, &D.count, b->cell_cap + 2);
@@ -375,15 +377,16 @@ extern "C" int sphgpu_batch_add(sphgpu_batch* b, const sphgpu_config* cfg, int64
     ACHK(hipMemcpyAsync(D.c + s0, hr.c.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
     ACHK(hipMemcpyAsync(D.p + s0, hr.p.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
     ACHK(hipMemcpyAsync(D.meta + s0, hr.meta.data(), n * sizeof(int4), hipMemcpyHostToDevice, b->stream));
-    ACHK(hipMemsetAsync(D.disp + s0, 0, nslot * sizeof(double4), b->stream));
     ACHK(hipMemsetAsync(D.div + s0, 0, nslot * sizeof(double2), b->stream));
     ACHK(hipMemsetAsync(D.ecnt + s0, 0, nslot * sizeof(int), b->stream));
     if (b->precision == SPHGPU_MIXED) {
         ACHK(hipMemcpyAsync(D.drho + s0, hr.drho.data(), n * sizeof(float), hipMemcpyHostToDevice, b->stream));
         ACHK(hipMemsetAsync(D.dvf + s0, 0, nslot * sizeof(float4), b->stream));
+        ACHK(hipMemsetAsync(D.dispf + s0, 0, nslot * sizeof(float4), b->stream));
         ACHK(hipMemsetAsync(D.def + s0, 0, nslot * sizeof(float), b->stream));
     } else {
         ACHK(hipMemsetAsync(D.dv + s0, 0, nslot * sizeof(double4), b->stream));
+        ACHK(hipMemsetAsync(D.disp + s0, 0, nslot * sizeof(double4), b->stream));
         ACHK(hipMemsetAsync(D.de + s0, 0, nslot * sizeof(double), b->stream));
     }
     // no candidate list yet: force the run's first build, clear its error words
@@ -429,15 +432,13 @@ static int rebuild_map(sphgpu_batch* b) {
 template <int DIM, bool MIXED>
 static void batch_chunk(sphgpu_batch* b, int m) {
     const int mc = (int)b->cells.high_water();
-    const RunMap M{b->d_runs, b->d_blk, b->d_runof, b->d_rows, (int)b->slots.high_water(), b->nact,
-                   b->D.start + mc};
+    const RunMap M{b->d_runs, b->d_blk, b->d_runof, b->d_rows, (int)b->slots.high_water(), b->nact};
     const Params P0{};
     Dev& D = b->D;
     hipStream_t st = b->stream;
     const int nbt = (int)b->blk.size();
     const int nr = (int)b->rows.size();
     const int nss = std::max(1, std::min((M.nslot + kBlock - 1) / kBlock, kStrideBlocks));
-    const int nsa = std::max(1, std::min((M.nact + kBlock - 1) / kBlock, kStrideBlocks));
     const int sb = (mc + kScanTile - 1) / kScanTile;
     const Gate G{b->d_vany};
     hipLaunchKernelGGL(k_bbox<true>, dim3(nbt), dim3(kBlock), 0, st, P0, D, M);
@@ -449,9 +450,8 @@ static void batch_chunk(sphgpu_batch* b, int m) {
         hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, st, D.bsum, sb, G);
         hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, D.start, G);
         hipLaunchKernelGGL(k_scatter<true>, dim3(nss), dim3(kBlock), 0, st, P0, D, G, M);
-        hipLaunchKernelGGL(k_cellsort, dim3(std::max(1, std::min((mc + kBlock - 1) / kBlock, kStrideBlocks))),
-                           dim3(kBlock), 0, st, D.start, D.perm, mc, G);
-        hipLaunchKernelGGL(k_cellpos<true>, dim3(nsa), dim3(kBlock), 0, st, P0, D, G, M);
+        hipLaunchKernelGGL(k_cellsort<true>, dim3(std::max(1, std::min((mc + kBlock - 1) / kBlock, kStrideBlocks))),
+                           dim3(kBlock), 0, st, P0, D, mc, G, M);
         hipLaunchKernelGGL((k_verlet_build<DIM, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, Gate{nullptr}, 0, M);
         hipLaunchKernelGGL((k_force_fused<DIM, MIXED, false, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, M);
         if (j + 1 < m) {
@@ -573,7 +573,7 @@ extern "C" int sphgpu_batch_download(sphgpu_batch* b, int32_t run, int32_t field
     const int64_t s0 = r->s0;
     auto off = [&](auto*& p) { if (p) p += s0; };
     off(V.posh); off(V.vm); off(V.rho); off(V.e); off(V.c); off(V.p); off(V.meta); off(V.disp);
-    off(V.dv); off(V.de); off(V.div); off(V.eta); off(V.drho); off(V.dvf);
+    off(V.dv); off(V.de); off(V.div); off(V.eta); off(V.drho); off(V.dvf); off(V.dispf);
     off(V.def);
     std::string msg;
     int rc = download_view(V, r->n, b->dim, b->precision == SPHGPU_MIXED, r->cfg, b->stream, field, dst, msg);

@@ -44,10 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_point_fast(Params P0, Dev D, int mod
                                                                                       : kErrNonFinite);
             D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
         }
-        const double4 b = D.xb[s];
-        const double ex = q.x - b.x, ey = q.y - b.y, ez = q.z - b.z;
-        const double d2 = ex * ex + ey * ey + ez * ez;
-        moved = !(d2 < P.disp_thr2);   // NaN also forces a rebuild
+        moved = moved_half_skin<MIXED>(P, D, s, q);
     }
     // one atomic per wave, and only when some lane crossed the threshold
     vote_rebuild(P, moved);
@@ -71,7 +68,8 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
     double4 qi = make_double4(0.0, 0.0, 0.0, 0.0);
     if (l < P.n) {
         qi = D.posh[s];
-        D.xb[s] = qi;
+        if (P.mixed) D.xbf[s] = make_float4((float)qi.x, (float)qi.y, (float)qi.z, 0.0f);
+        else D.xb[s] = qi;
         const int4 mt = D.meta[s];
         active = mt.w < P.n_real && mt.y != -1;
         if (mt.w < P.n_real && mt.y == -1) D.vcnt[s] = 0;
@@ -425,14 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
             v.x = vp.x + fac * dvx;
             v.y = vp.y + fac * dvy;
             if (DIM == 3) v.z = vp.z + fac * dvz;
-            double4 ds = D.disp[s];
-            if (first) {
-                ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
-            } else {
-                ds.x = ds.x + dt * v.x; ds.y = ds.y + dt * v.y;
-                if (DIM == 3) ds.z = ds.z + dt * v.z;
-            }
-            D.disp[s] = ds;
+            add_disp<DIM, MIXED>(D, s, first, dt, v);
             q.x = q.x + dt * v.x;
             q.y = q.y + dt * v.y;
             if (DIM == 3) q.z = q.z + dt * v.z;
@@ -447,9 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
                 atomicOr(P.err, kErrNonFinite);
             if (MIXED) eos_mixed_vals(P, D, s, mt.x, dr_pred, v, q);
             else eos_th(P, D, s, mt);
-            const double4 b = D.xb[s];
-            const double ex = q.x - b.x, ey = q.y - b.y, ez = q.z - b.z;
-            moved = !(ex * ex + ey * ey + ez * ez < P.disp_thr2);
+            moved = moved_half_skin<MIXED>(P, D, s, q);
         }
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
     }

@@ -54,6 +54,7 @@ struct Params {
     // when some particle has moved skin/2 since the last build
     double skin;
     double disp_thr2;   // (skin/2)^2, slightly reduced
+    double disp_half;   // skin/2, slightly reduced
     int verlet_cap;
     double hmax;        // largest smoothing length
     // where this run lives: a single-run context has s0 = 0, c0 = 0; a run of
@@ -100,7 +101,6 @@ struct RunMap {
     const int* rows;      // rows of the active runs (one block each in k_grid)
     int nslot;            // slots below the arena's high-water mark
     int nact;             // particles of the runnable runs
-    const int* nsorted;   // entries of the last cell sort (written by the scan)
 };
 
 template <bool BATCH>

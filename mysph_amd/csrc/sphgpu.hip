@@ -93,6 +93,9 @@ struct Dev {
     float* def;       // de/dt
     // fluid fast path
     double4* xb;      // positions at the last candidate-list build
+    // mixed precision keeps two output/bookkeeping records in FP32 instead
+    float4* dispf;    // displacement (FP64 disp otherwise)
+    float4* xbf;      // positions at the last build (FP64 xb otherwise)
     int* vnbr;        // candidate list, column-major like nbr
     int* vcnt;
     int* ecnt;        // accepted neighbours per particle in the last force pass
@@ -121,6 +124,52 @@ __device__ __forceinline__ void bbox_block_reduce(double mn[3], double mx[3], do
         for (int w = 1; w < kBlock / 64; ++w) { a = fmin(a, smn[d][w]); b = fmax(b, smx[d][w]); }
         partial[blk * 6 + d] = a;
         partial[blk * 6 + 3 + d] = b;
+    }
+}
+
+// Displacement of the leap-frog correct (timeIntegration.f90:195-212): dt v
+// on the first step, accumulated after. FP32 in mixed precision, where it is
+// an output-only field.
+template <int DIM, bool MIXED>
+__device__ __forceinline__ void add_disp(const Dev& D, int s, bool first, double dt, const double4& v) {
+    if constexpr (MIXED) {
+        float4 ds = D.dispf[s];
+        const float ax = (float)(dt * v.x), ay = (float)(dt * v.y), az = DIM == 3 ? (float)(dt * v.z) : 0.0f;
+        if (first) {
+            ds.x = ax; ds.y = ay; ds.z = az;
+        } else {
+            ds.x = ds.x + ax; ds.y = ds.y + ay;
+            if (DIM == 3) ds.z = ds.z + az;
+        }
+        D.dispf[s] = ds;
+    } else {
+        double4 ds = D.disp[s];
+        if (first) {
+            ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
+        } else {
+            ds.x = ds.x + dt * v.x; ds.y = ds.y + dt * v.y;
+            if (DIM == 3) ds.z = ds.z + dt * v.z;
+        }
+        D.disp[s] = ds;
+    }
+}
+
+// Has the particle moved skin/2 since the last candidate-list build? Mixed
+// precision keeps the build positions in FP32; its threshold is lowered by
+// their rounding error (2^-24 |x| per axis, doubled), so a true move of
+// skin/2 always votes. A NaN position votes too.
+template <bool MIXED>
+__device__ __forceinline__ bool moved_half_skin(const Params& P, const Dev& D, int s, const double4& q) {
+    if constexpr (MIXED) {
+        const float4 b = D.xbf[s];
+        const double ex = q.x - (double)b.x, ey = q.y - (double)b.y, ez = q.z - (double)b.z;
+        const double xm = fmax(fabs((double)b.x), fmax(fabs((double)b.y), fabs((double)b.z)));
+        const double half = P.disp_half - 2.0 * 1.7320508075688772 * 5.9604644775390625e-8 * xm;
+        return !(half > 0.0 && ex * ex + ey * ey + ez * ez < half * half);
+    } else {
+        const double4 b = D.xb[s];
+        const double ex = q.x - b.x, ey = q.y - b.y, ez = q.z - b.z;
+        return !(ex * ex + ey * ey + ez * ez < P.disp_thr2);
     }
 }
 
@@ -358,41 +407,36 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D, Gate G, Run
     }
 }
 
-// Ascending slot order inside each cell: the pair order, and so every sum,
-// is the same from run to run.
-__global__ __launch_bounds__(kBlock) void k_cellsort(const int* __restrict__ start, int* perm,
-                                                     int m, Gate G) {
+// Ascending slot order inside each cell (the pair order, and so every sum, is
+// the same from run to run), then the FP32 positions (relative to the run's
+// grid origin) in cell order with the slot in w: the candidate loop of the
+// search streams these 16-byte records. One thread per cell.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_cellsort(Params P0, Dev D, int m, Gate G, RunMap M) {
     if (!gate_open(G)) return;
     for (int c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
-    int a = start[c], b = start[c + 1];
-    for (int i = a + 1; i < b; ++i) {
-        int v = perm[i];
-        int j = i - 1;
-        while (j >= a && perm[j] > v) {
-            perm[j + 1] = perm[j];
-            --j;
+        const int a = D.start[c], b = D.start[c + 1];
+        if (a == b) continue;
+        int* perm = D.perm;
+        for (int i = a + 1; i < b; ++i) {
+            int v = perm[i];
+            int j = i - 1;
+            while (j >= a && perm[j] > v) {
+                perm[j + 1] = perm[j];
+                --j;
+            }
+            perm[j + 1] = v;
         }
-        perm[j + 1] = v;
-    }
-    }
-}
-
-// FP32 positions (relative to the grid origin) in cell order, with the slot
-// in w: the candidate loop of the search streams these 16-byte records.
-template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_cellpos(Params P0, Dev D, Gate G, RunMap M) {
-    if (!gate_open(G)) return;
-    const int lim = BATCH ? *M.nsorted : P0.n;
-    for (int k = blockIdx.x * kBlock + threadIdx.x; k < lim; k += gridDim.x * kBlock) {
-        int j = D.perm[k];
-        const Params& P = BATCH ? M.runs[M.runof[j]] : P0;
-        double4 q = D.posh[j];
-        D.cpos[k] = make_float4((float)(q.x - P.org[0]), (float)(q.y - P.org[1]),
-                                (float)(q.z - P.org[2]), __int_as_float(j));
+        const Params& P = BATCH ? M.runs[M.runof[perm[a]]] : P0;   // one run per cell
+        for (int k = a; k < b; ++k) {
+            const int j = perm[k];
+            const double4 q = D.posh[j];
+            D.cpos[k] = make_float4((float)(q.x - P.org[0]), (float)(q.y - P.org[1]),
+                                    (float)(q.z - P.org[2]), __int_as_float(j));
+        }
     }
 }
 
-// Leap-frog predict of one real particle (timeIntegration.f90:137-170).
 template <int DIM>
 __device__ __forceinline__ void predict_one(const Params& P, const Dev& D, int s, int4 mt,
                                             bool first) {
@@ -1180,14 +1224,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(Params P0, Dev D, int first_
             v.y = vp.y + fac * dvy;
             if (DIM == 3) v.z = vp.z + fac * dvz;
             D.vm[s] = v;
-            double4 ds = D.disp[s];
-            if (first) {
-                ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
-            } else {
-                ds.x = ds.x + dt * v.x; ds.y = ds.y + dt * v.y;
-                if (DIM == 3) ds.z = ds.z + dt * v.z;
-            }
-            D.disp[s] = ds;
+            add_disp<DIM, MIXED>(D, s, first, dt, v);
             q.x = q.x + dt * v.x;
             q.y = q.y + dt * v.y;
             if (DIM == 3) q.z = q.z + dt * v.z;
@@ -1467,6 +1504,7 @@ static void plan_grid_params(Params& P, int dim, bool verlet, int64_t n, const d
     P.hmax = hmax;
     const double half = 0.5 * P.skin * (1.0 - 1e-9);
     P.disp_thr2 = half * half;
+    P.disp_half = half;
     double cs = (cut_max + P.skin) / P.win * (1.0 + 1e-6);
     if (!(cs > 0)) cs = 1.0;
     double lo[3], ext[3];
@@ -1521,9 +1559,8 @@ static void launch_sort(sphgpu_ctx* c, Gate G = Gate{nullptr}, int with_ref = 1)
     hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
                        c->D.start, G);
     hipLaunchKernelGGL(k_scatter<false>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, RunMap{});
-    hipLaunchKernelGGL(k_cellsort, dim3(std::min((m + kBlock - 1) / kBlock, kStrideBlocks)), dim3(kBlock),
-                       0, c->stream, c->D.start, c->D.perm, m, G);
-    hipLaunchKernelGGL(k_cellpos<false>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, RunMap{});
+    hipLaunchKernelGGL(k_cellsort<false>, dim3(std::min((m + kBlock - 1) / kBlock, kStrideBlocks)), dim3(kBlock),
+                       0, c->stream, P, c->D, m, G, RunMap{});
 }
 
 static Gate fast_gate(const sphgpu_ctx* c) { return Gate{c->D.vflag}; }
@@ -1680,13 +1717,18 @@ static int alloc_state(sphgpu_ctx* c) {
     int rc = 0;
     rc |= dalloc(c, &D.posh, N); rc |= dalloc(c, &D.vm, N); rc |= dalloc(c, &D.rho, N);
     rc |= dalloc(c, &D.e, N); rc |= dalloc(c, &D.c, N); rc |= dalloc(c, &D.p, N);
-    rc |= dalloc(c, &D.meta, N); rc |= dalloc(c, &D.disp, N); rc |= dalloc(c, &D.dv, N);
+    rc |= dalloc(c, &D.meta, N); rc |= dalloc(c, &D.dv, N);
+    if (c->P.mixed) {
+        rc |= dalloc(c, &D.dispf, N); rc |= dalloc(c, &D.xbf, N);
+    } else {
+        rc |= dalloc(c, &D.disp, N); rc |= dalloc(c, &D.xb, N);
+    }
     rc |= dalloc(c, &D.de, N); rc |= dalloc(c, &D.vprev, N); rc |= dalloc(c, &D.eprev, N);
     rc |= dalloc(c, &D.th, N); rc |= dalloc(c, &D.div, N); rc |= dalloc(c, &D.eta, N);
     rc |= dalloc(c, &D.tds, N); rc |= dalloc(c, &D.key, N); rc |= dalloc(c, &D.refcell, N);
     rc |= dalloc(c, &D.perm, N); rc |= dalloc(c, &D.ncnt, N);
     rc |= dalloc(c, &D.cpos, N);
-    rc |= dalloc(c, &D.xb, N); rc |= dalloc(c, &D.vcnt, N); rc |= dalloc(c, &D.ecnt, N);
+    rc |= dalloc(c, &D.vcnt, N); rc |= dalloc(c, &D.ecnt, N);
     rc |= dalloc(c, &D.vflag, 2);
     bool need_tensor = c->cfg.viscosity || c->P.has_solid;
     if (need_tensor) rc |= dalloc(c, &D.Ep, T);
@@ -1935,7 +1977,8 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     Dev& D = c->D;
     HIPCHK(c, hipMemcpyAsync(D.posh, posh.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.vm, vm.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(D.disp, disp.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    if (!c->P.mixed)   // FP32 dispf starts at zero from its allocation
+        HIPCHK(c, hipMemcpyAsync(D.disp, disp.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.dv, zero4.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.rho, vrho.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.e, ve.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -2103,6 +2146,7 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         case SPHGPU_F_DVDT: return getf4(D.dvf, 0);
         case SPHGPU_F_DRHODT: return getf4(D.dvf, 1);
         case SPHGPU_F_RHO: return getf1(D.drho, true);
+        case SPHGPU_F_DISP: return getf4(D.dispf, 0);
         case SPHGPU_F_DEDT: return getf1(D.def, false);
         default: break;
         }
