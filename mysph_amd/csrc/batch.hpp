@@ -102,7 +102,7 @@ int64_t batch_bytes(int dim, int precision, int64_t slots, int64_t cells, int ma
     b += N * (sizeof(float4));                         // cpos
     b += N * (int64_t)vcap * sizeof(int);              // candidate lists
     if (precision == SPHGPU_MIXED)
-        b += N * (sizeof(float4) * 6 + sizeof(float) * 3);   // posf vmf thf dvf dispf xbf, drho drho_prev def
+        b += N * (sizeof(float4) * 6 + sizeof(float) * 5 + sizeof(float2));   // + ef eprevf divf
     else
         b += N * (sizeof(double4) * 4 + sizeof(double));     // dv th disp xb, de
     b += (N / kBlock + 1) * 6 * sizeof(double) + (N / kBlock + 1) * sizeof(int2);
@@ -230,6 +230,7 @@ extern "C" int sphgpu_batch_create(int32_t dim, int32_t precision, int64_t slot_
     rc |= balloc(b, &D.vnbr, (size_t)N * b->verlet_cap);
     if (precision == SPHGPU_MIXED) {
         rc |= balloc(b, &D.dispf, N); rc |= balloc(b, &D.xbf, N);
+        rc |= balloc(b, &D.ef, N); rc |= balloc(b, &D.eprevf, N); rc |= balloc(b, &D.divf, N);
         rc |= balloc(b, &D.vmf, N); rc |= balloc(b, &D.thf, N); rc |= balloc(b, &D.posf, N);
         rc |= balloc(b, &D.dvf, N); rc |= balloc(b, &D.drho, N); rc |= balloc(b, &D.drho_prev, N);
         rc |= balloc(b, &D.def, N);
@@ -383,6 +384,12 @@ extern "C" int sphgpu_batch_add(sphgpu_batch* b, const sphgpu_config* cfg, int64
         ACHK(hipMemcpyAsync(D.drho + s0, hr.drho.data(), n * sizeof(float), hipMemcpyHostToDevice, b->stream));
         ACHK(hipMemsetAsync(D.dvf + s0, 0, nslot * sizeof(float4), b->stream));
         ACHK(hipMemsetAsync(D.dispf + s0, 0, nslot * sizeof(float4), b->stream));
+        ACHK(hipMemsetAsync(D.divf + s0, 0, nslot * sizeof(float2), b->stream));
+        {
+            std::vector<float> ef(hr.e.begin(), hr.e.end());
+            ACHK(hipMemcpyAsync(D.ef + s0, ef.data(), n * sizeof(float), hipMemcpyHostToDevice, b->stream));
+            ACHK(hipStreamSynchronize(b->stream));
+        }
         ACHK(hipMemsetAsync(D.def + s0, 0, nslot * sizeof(float), b->stream));
     } else {
         ACHK(hipMemsetAsync(D.dv + s0, 0, nslot * sizeof(double4), b->stream));
@@ -446,9 +453,10 @@ static void batch_chunk(sphgpu_batch* b, int m) {
     hipLaunchKernelGGL((k_point_fast<DIM, MIXED, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, 0, M);
     for (int j = 0; j < m; ++j) {
         hipLaunchKernelGGL((k_keys<DIM, true>), dim3(nss), dim3(kBlock), 0, st, P0, D, G, 0, M);
+        const int own = sb <= kOwnPrefixTiles;
         hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, G);
-        hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, st, D.bsum, sb, G);
-        hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, D.start, G);
+        if (!own) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, st, D.bsum, sb, G);
+        hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, st, D.count, mc, D.bsum, D.start, own, G);
         hipLaunchKernelGGL(k_scatter<true>, dim3(nss), dim3(kBlock), 0, st, P0, D, G, M);
         hipLaunchKernelGGL(k_cellsort<true>, dim3(std::max(1, std::min((mc + kBlock - 1) / kBlock, kStrideBlocks))),
                            dim3(kBlock), 0, st, P0, D, mc, G, M);
@@ -574,6 +582,7 @@ extern "C" int sphgpu_batch_download(sphgpu_batch* b, int32_t run, int32_t field
     auto off = [&](auto*& p) { if (p) p += s0; };
     off(V.posh); off(V.vm); off(V.rho); off(V.e); off(V.c); off(V.p); off(V.meta); off(V.disp);
     off(V.dv); off(V.de); off(V.div); off(V.eta); off(V.drho); off(V.dvf); off(V.dispf);
+    off(V.ef); off(V.divf);
     off(V.def);
     std::string msg;
     int rc = download_view(V, r->n, b->dim, b->precision == SPHGPU_MIXED, r->cfg, b->stream, field, dst, msg);

@@ -365,7 +365,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
         D.dv[s] = make_double4(ax, ay, az, drho);
         D.de[s] = de;
     }
-    D.div[s] = make_double2(-(double)sx, -(double)sv);
+    if (MIXED) D.divf[s] = make_float2(-(float)sx, -(float)sv);
+    else D.div[s] = make_double2(-(double)sx, -(double)sv);
     D.ecnt[s] = acc_n;
 }
 
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
             // not stored (the pair pass reads its FP32 copy; the closing
             // correct writes the FP64 state)
             double4 v = MIXED ? make_double4(0.0, 0.0, 0.0, vp.w) : D.vm[s];
-            double e, rho_c = 0.0;
+            double rho_c = 0.0;
             float dr_pred = 0.0f;
             if (MIXED) {
                 const float4 dv = D.dvf[s];
@@ -414,12 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
                 finite_rho = isfinite(rho_c);
                 D.rho[s] = rho_c + h2 * dv.w;                   // predict of the next step
             }
-            e = D.eprev[s] + fac * dee;
-            if (e < 0) e = 0;
-            D.eprev[s] = e;
-            e = e + h2 * dee;
-            if (e < 0) e = 0;
-            D.e[s] = e;
+            energy_correct_predict<MIXED>(D, s, fac, h2, dee);
             v.x = vp.x + fac * dvx;
             v.y = vp.y + fac * dvy;
             if (DIM == 3) v.z = vp.z + fac * dvz;

@@ -96,6 +96,9 @@ struct Dev {
     // mixed precision keeps two output/bookkeeping records in FP32 instead
     float4* dispf;    // displacement (FP64 disp otherwise)
     float4* xbf;      // positions at the last build (FP64 xb otherwise)
+    float* ef;        // internal energy (FP64 e otherwise)
+    float* eprevf;    // its value at the last correct (FP64 eprev otherwise)
+    float2* divf;     // divergences of position and velocity (FP64 div otherwise)
     int* vnbr;        // candidate list, column-major like nbr
     int* vcnt;
     int* ecnt;        // accepted neighbours per particle in the last force pass
@@ -151,6 +154,29 @@ __device__ __forceinline__ void add_disp(const Dev& D, int s, bool first, double
             if (DIM == 3) ds.z = ds.z + dt * v.z;
         }
         D.disp[s] = ds;
+    }
+}
+
+// Internal energy of the leap-frog correct and the half-step predict that
+// follows it, clamped at zero (timeIntegration.f90:147-149, 186-188, 224-226);
+// FP32 in mixed precision. Returns nothing: both values are stored.
+template <bool MIXED>
+__device__ __forceinline__ void energy_correct_predict(const Dev& D, int s, double fac, double h2,
+                                                       double dee) {
+    if constexpr (MIXED) {
+        float e = D.eprevf[s] + (float)fac * (float)dee;
+        if (e < 0) e = 0;
+        D.eprevf[s] = e;
+        e = e + (float)h2 * (float)dee;
+        if (e < 0) e = 0;
+        D.ef[s] = e;
+    } else {
+        double e = D.eprev[s] + fac * dee;
+        if (e < 0) e = 0;
+        D.eprev[s] = e;
+        e = e + h2 * dee;
+        if (e < 0) e = 0;
+        D.e[s] = e;
     }
 }
 
@@ -359,10 +385,20 @@ __global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb, Gate G) {
 }
 
 // Cell start table: start[c] = exclusive prefix of count; the tile goes through
-// LDS so that global loads and stores stay coalesced.
+// LDS so that global loads and stores stay coalesced. With own_prefix the
+// block adds up the raw tile totals of k_scan1 before it (cheap while there
+// are few tiles, and one launch fewer); otherwise bsum was scanned by k_scan2.
 __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count, int m,
-                                                  const int* bsum, int* start, Gate G) {
+                                                  const int* bsum, int* start, int own_prefix, Gate G) {
     if (!gate_open(G)) return;
+    int base_off = 0;
+    if (own_prefix) {
+        int acc = 0;
+        for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) acc += bsum[b];
+        block_exclusive_scan(acc, &base_off);
+    } else {
+        base_off = bsum[blockIdx.x];
+    }
     __shared__ int tile[kScanTile];
     const int base = blockIdx.x * kScanTile;
     for (int k = 0; k < kScanItems; ++k) {
@@ -377,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count,
         s += v[k];
     }
     int tot;
-    int ex = block_exclusive_scan(s, &tot) + bsum[blockIdx.x];
+    int ex = block_exclusive_scan(s, &tot) + base_off;
     for (int k = 0; k < kScanItems; ++k) {
         tile[threadIdx.x * kScanItems + k] = ex;
         ex += v[k];
@@ -387,8 +423,15 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count,
         int i = threadIdx.x + k * kBlock;
         if (base + i < m) start[base + i] = tile[i];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) start[m] = bsum[gridDim.x];
+    if (threadIdx.x == 0) {
+        if (own_prefix && blockIdx.x == gridDim.x - 1) start[m] = base_off + tot;
+        if (!own_prefix && blockIdx.x == 0) start[m] = bsum[gridDim.x];
+    }
 }
+
+// Tiles up to which k_scan3 adds up its own prefix (its work grows with the
+// square of the tile count).
+constexpr int kOwnPrefixTiles = 4096;
 
 // Place each particle in its cell. The decrement leaves every count at zero
 // again for the next step.
@@ -491,14 +534,14 @@ template <int DIM>
 __device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int s, bool first) {
     double4 v = D.vm[s];
     float dr = D.drho[s];
-    double e = D.e[s];
+    float e = D.ef[s];
     D.vprev[s] = make_double4(v.x, v.y, v.z, v.w);   // mixed: w holds the mass
     D.drho_prev[s] = dr;
-    D.eprev[s] = e;
+    D.eprevf[s] = e;
     if (first) return;
     const double h2 = P.dt / 2;
     float4 dv = D.dvf[s];
-    e = e + h2 * (double)D.def[s];
+    e = e + (float)h2 * D.def[s];
     if (e < 0) e = 0;
     dr = dr + (float)h2 * dv.w;
     v.x = v.x + h2 * (double)dv.x;
@@ -506,7 +549,7 @@ __device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int
     if (DIM == 3) v.z = v.z + h2 * (double)dv.z;
     D.vm[s] = v;
     D.drho[s] = dr;
-    D.e[s] = e;
+    D.ef[s] = e;
 }
 
 // FP32 EOS on the density deviation: the Tait term (rho/rho0)^gamma - 1 is
@@ -1180,7 +1223,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
         D.dv[s] = make_double4(ax, ay, az, drho);
         D.de[s] = de;
     }
-    D.div[s] = make_double2(-(double)sx, -(double)sv);
+    if (MIXED) D.divf[s] = make_float2(-(float)sx, -(float)sv);
+    else D.div[s] = make_double2(-(double)sx, -(double)sv);
 }
 
 #include "fluid_fast.hpp"
@@ -1216,9 +1260,15 @@ __global__ __launch_bounds__(kBlock) void k_correct(Params P0, Dev D, int first_
                 if (!P.summation_density) D.rho[s] = vp.w + fac * dv.w;
                 finite_rho = isfinite(D.rho[s]);
             }
-            double e = D.eprev[s] + fac * dee;
-            if (e < 0) e = 0;
-            D.e[s] = e;
+            if (MIXED) {
+                float e = D.eprevf[s] + (float)fac * (float)dee;
+                if (e < 0) e = 0;
+                D.ef[s] = e;
+            } else {
+                double e = D.eprev[s] + fac * dee;
+                if (e < 0) e = 0;
+                D.e[s] = e;
+            }
             double4 v = D.vm[s];
             v.x = vp.x + fac * dvx;
             v.y = vp.y + fac * dvy;
@@ -1554,10 +1604,11 @@ static void launch_sort(sphgpu_ctx* c, Gate G = Gate{nullptr}, int with_ref = 1)
     hipLaunchKernelGGL((k_keys<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, with_ref, RunMap{});
     const int m = (int)P.ncell + 1;
     const int sb = (m + kScanTile - 1) / kScanTile;
+    const int own = sb <= kOwnPrefixTiles;
     hipLaunchKernelGGL(k_scan1, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum, G);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb, G);
+    if (!own) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kBlock), 0, c->stream, c->D.bsum, sb, G);
     hipLaunchKernelGGL(k_scan3, dim3(sb), dim3(kBlock), 0, c->stream, c->D.count, m, c->D.bsum,
-                       c->D.start, G);
+                       c->D.start, own, G);
     hipLaunchKernelGGL(k_scatter<false>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, RunMap{});
     hipLaunchKernelGGL(k_cellsort<false>, dim3(std::min((m + kBlock - 1) / kBlock, kStrideBlocks)), dim3(kBlock),
                        0, c->stream, P, c->D, m, G, RunMap{});
@@ -1720,6 +1771,7 @@ static int alloc_state(sphgpu_ctx* c) {
     rc |= dalloc(c, &D.meta, N); rc |= dalloc(c, &D.dv, N);
     if (c->P.mixed) {
         rc |= dalloc(c, &D.dispf, N); rc |= dalloc(c, &D.xbf, N);
+        rc |= dalloc(c, &D.ef, N); rc |= dalloc(c, &D.eprevf, N); rc |= dalloc(c, &D.divf, N);
     } else {
         rc |= dalloc(c, &D.disp, N); rc |= dalloc(c, &D.xb, N);
     }
@@ -1982,6 +2034,11 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     HIPCHK(c, hipMemcpyAsync(D.dv, zero4.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.rho, vrho.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.e, ve.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<float> vef;
+    if (c->P.mixed) {
+        vef.assign(ve.begin(), ve.end());
+        HIPCHK(c, hipMemcpyAsync(D.ef, vef.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(c, hipMemcpyAsync(D.c, vc.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.p, vp.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.meta, meta.data(), n * sizeof(int4), hipMemcpyHostToDevice, c->stream));
@@ -2147,6 +2204,15 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         case SPHGPU_F_DRHODT: return getf4(D.dvf, 1);
         case SPHGPU_F_RHO: return getf1(D.drho, true);
         case SPHGPU_F_DISP: return getf4(D.dispf, 0);
+        case SPHGPU_F_E: return getf1(D.ef, false);
+        case SPHGPU_F_DIVX:
+        case SPHGPU_F_DIVV: {
+            std::vector<float2> h(n);
+            VCHK(hipMemcpyAsync(h.data(), D.divf, n * sizeof(float2), hipMemcpyDeviceToHost, stream));
+            VCHK(hipStreamSynchronize(stream));
+            for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = field == SPHGPU_F_DIVX ? h[s].x : h[s].y;
+            return 0;
+        }
         case SPHGPU_F_DEDT: return getf1(D.def, false);
         default: break;
         }
