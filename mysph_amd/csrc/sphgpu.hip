@@ -1367,6 +1367,8 @@ struct sphgpu_ctx {
     bool force_general = false;   // testing switch: use the general force kernel
     bool use_verlet = true;       // fluid fast path: candidate list + fused pass
     bool fused_done = false;      // a fused pass has run since the upload: candidate list valid
+    bool use_graph = true;        // replay captured step sequences (SPHGPU_NO_GRAPH=1: eager)
+    hipGraphExec_t graph = nullptr;   // kGraphSteps fast-path steps, valid until the next upload
     std::string err;
     int* h_err = nullptr;   // pinned
 };
@@ -1460,6 +1462,8 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
     c->force_general = fg && fg[0] == '1';
     const char* nv = getenv("SPHGPU_NO_VERLET");
     c->use_verlet = !(nv && nv[0] == '1');
+    const char* ng = getenv("SPHGPU_NO_GRAPH");
+    c->use_graph = !(ng && ng[0] == '1');
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         c->stream = nullptr;
@@ -1655,38 +1659,107 @@ static void launch_fast(sphgpu_ctx* c, int mode, int count_only = 0) {
     launch_fast_tail<DIM>(c, count_only);
 }
 
-// nsteps leap-frog steps on the fast path: the correct of step n-1 is fused
-// with the predict of step n; the last step ends with a plain correct.
+// One step of the fast path that is followed by another: list rebuild if due,
+// fused force pass, correct of this step fused with the predict + EOS of the
+// next, and the next step's reference grid and cells.
 template <int DIM>
-static void advance_fast(sphgpu_ctx* c, int nsteps) {
+static void fast_body_step(sphgpu_ctx* c, bool timed) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const int nbs = std::min(nb, kStrideBlocks);
-    launch_fast_head<DIM>(c, c->step == 1 ? 1 : 0);
-    for (int k = 0; k < nsteps; ++k) {
-        c->time_force = (k == nsteps - 1);
-        launch_fast_tail<DIM>(c, 0);
-        const int first = c->step == 1;
-        if (k + 1 < nsteps) {
-            if (P.mixed)
-                hipLaunchKernelGGL((k_correct_predict<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
-            else
-                hipLaunchKernelGGL((k_correct_predict<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
-            hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
-            hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
-        } else if (P.mixed) {
-            hipLaunchKernelGGL((k_correct<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
-        } else {
-            hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
-        }
+    c->time_force = timed;
+    launch_fast_tail<DIM>(c, 0);
+    const int first = c->step == 1;
+    if (P.mixed)
+        hipLaunchKernelGGL((k_correct_predict<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
+    else
+        hipLaunchKernelGGL((k_correct_predict<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+    c->time_force = false;
+}
+
+// The last step of a call: force pass and a plain correct.
+template <int DIM>
+static void fast_last_step(sphgpu_ctx* c, bool timed) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    c->time_force = timed;
+    launch_fast_tail<DIM>(c, 0);
+    const int first = c->step == 1;
+    if (P.mixed)
+        hipLaunchKernelGGL((k_correct<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
+    else
+        hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
+    c->time_force = false;
+}
+
+// Steps per captured graph of the fast path.
+constexpr int kGraphSteps = 10;
+
+static void drop_graph(sphgpu_ctx* c) {
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+}
+
+// Capture kGraphSteps body steps (kernel nodes only: no memsets, no events).
+// The kernels take the uploaded Params and device arrays by value, and a body
+// step after the first has the same arguments every time, so the graph holds
+// until the next upload.
+template <int DIM>
+static int capture_body(sphgpu_ctx* c) {
+    const int64_t step = c->step;
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphSteps; ++k) {
+        fast_body_step<DIM>(c, false);
         c->step++;
     }
-    c->time_force = false;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->step = step;
+    if (e != hipSuccess) return fail(c, SPHGPU_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->graph = nullptr;
+        return fail(c, SPHGPU_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// nsteps leap-frog steps on the fast path: predict + EOS of the first step,
+// then nsteps - 1 body steps (whole groups of kGraphSteps replay a captured
+// graph once the first step is past) and a closing step whose force kernel
+// is timed.
+template <int DIM>
+static int advance_fast(sphgpu_ctx* c, int nsteps) {
+    launch_fast_head<DIM>(c, c->step == 1 ? 1 : 0);
+    int body = nsteps - 1;
+    if (c->use_graph && c->step > 1 && body >= kGraphSteps) {
+        if (!c->graph) {
+            int rc = capture_body<DIM>(c);
+            if (rc) return rc;
+        }
+        const int reps = body / kGraphSteps;
+        for (int r = 0; r < reps; ++r) {
+            HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+            c->step += kGraphSteps;
+        }
+        body -= reps * kGraphSteps;
+    }
+    for (int k = 0; k < body; ++k) {
+        fast_body_step<DIM>(c, false);
+        c->step++;
+    }
+    fast_last_step<DIM>(c, true);
+    c->step++;
+    return 0;
 }
 
 static void force_rebuild(sphgpu_ctx* c) {
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, c->stream, c->D.vflag);
 }
+
 
 // One evaluation of the derivatives (mode bits as k_point_build).
 template <int DIM>
@@ -2052,6 +2125,7 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     c->step = 1;
     c->has_state = true;
     c->fused_done = false;
+    drop_graph(c);
     rc = dim == 2 ? size_neighbour_list<2>(c) : size_neighbour_list<3>(c);
     if (rc) { c->has_state = false; return rc; }
     // the grid partials of the uploaded positions seed the first step
@@ -2088,8 +2162,8 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
     double force_ms = 0;
     int64_t force_n = 0;
     if (fluid_fast_path(c) && c->use_verlet && nsteps > 0) {
-        if (c->dim == 2) advance_fast<2>(c, nsteps);
-        else advance_fast<3>(c, nsteps);
+        int rc = c->dim == 2 ? advance_fast<2>(c, nsteps) : advance_fast<3>(c, nsteps);
+        if (rc) return rc;
     } else {
         for (int k = 0; k < nsteps; ++k) {
             c->time_force = (k == nsteps - 1);
@@ -2358,6 +2432,7 @@ extern "C" const char* sphgpu_last_error(sphgpu_ctx* c) {
 extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graph(c);
     free_all(c);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);

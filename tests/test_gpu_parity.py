@@ -289,6 +289,30 @@ def test_fast_path_advance_in_pieces_equals_one_call(precision):
             assert np.array_equal(a.get(k), b.get(k)), k
 
 
+@pytest.mark.parametrize("precision", [C.FP64, C.MIXED])
+def test_graph_replays_equal_eager_launches(precision, monkeypatch):
+    """Groups of steps replay a captured hipGraph; every replay, with
+    downloads and synchronisation in between, equals eager launches bit for
+    bit (rebuilds of the candidate list included)."""
+    case = cases.dam_break_2d(dx=0.04)
+    nt = case["n_real"]
+    rng = np.random.default_rng(21)
+    case["v"] = np.zeros_like(case["x"])
+    case["v"][:nt] = rng.normal(0.0, 1.0, (nt, 2))
+    cfg = case["config"].copy()
+    cfg.precision = precision
+    with Simulation(case, cfg) as g:
+        monkeypatch.setenv("SPHGPU_NO_GRAPH", "1")
+        with Simulation(case, cfg) as e:
+            for sim in (g, e):
+                sim.advance(7)
+            for rep in range(3):
+                for sim in (g, e):
+                    sim.advance(34)          # 3 graph replays + 4 eager steps
+                for k in STATE_FIELDS:
+                    assert np.array_equal(g.get(k), e.get(k)), (rep, k)
+
+
 def test_exact_list_path_matches_fast_path(monkeypatch):
     case = cases.dam_break_3d(dx=0.03)
     nt = case["n_real"]
