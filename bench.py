@@ -43,7 +43,7 @@ def measured_traffic(mixed):
             table = json.load(f)
     except (OSError, ValueError):
         return None
-    tag = "k_force_fused<3, %s, false>" % ("true" if mixed else "false")
+    tag = "k_force_fused<3, %s, false, false>" % ("true" if mixed else "false")
     for name, rec in table.items():
         if tag in name and rec.get("fetch_bytes") and rec.get("write_bytes") is not None:
             return rec["fetch_bytes"] + rec["write_bytes"]
