@@ -231,7 +231,7 @@ extern "C" int sphgpu_batch_create(int32_t dim, int32_t precision, int64_t slot_
     if (precision == SPHGPU_MIXED) {
         rc |= balloc(b, &D.dispf, N); rc |= balloc(b, &D.xbf, N);
         rc |= balloc(b, &D.ef, N); rc |= balloc(b, &D.eprevf, N); rc |= balloc(b, &D.divf, N);
-        rc |= balloc(b, &D.vmf, N); rc |= balloc(b, &D.thf, N); rc |= balloc(b, &D.posf, N);
+        rc |= balloc(b, &D.prf, N); rc |= balloc(b, &D.posf, N);
         rc |= balloc(b, &D.dvf, N); rc |= balloc(b, &D.drho, N); rc |= balloc(b, &D.drho_prev, N);
         rc |= balloc(b, &D.def, N);
     } else {

@@ -190,8 +190,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     bool esc_i = false;
     float lof = 0.0f, hif = 0.0f;
     if (MIXED) {
-        vi = *(const R4*)&D.vmf[s];
-        thi = *(const R4*)&D.thf[s];
+        vi = *(const R4*)&D.prf[s].vm;
+        thi = *(const R4*)&D.prf[s].th;
         const float4 pf = D.posf[s];
         pix = pf.x; piy = pf.y; piz = pf.z;
         const uint32_t pk = __float_as_uint(pf.w);
@@ -231,68 +231,10 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     int acc_n = 0;
     const int n = (mt.y == -1) ? 0 : min(D.vcnt[s], P.verlet_cap);
     const int* __restrict__ nl = D.vnbr + s;
-    int jn = n > 0 ? nl[0] : 0;
-    for (int k = 0; k < n; ++k) {
-        const int j = jn;
-        if (k + 1 < n) jn = nl[(int64_t)(k + 1) * P.ldn];
-        R dx, dy, dz, r2;
-        if (MIXED) {
-            const float4 pf = D.posf[j];
-            const uint32_t pk = __float_as_uint(pf.w);
-            int cxj, cyj, czj;
-            unpack_cell<DIM>(pk, cxj, cyj, czj);
-            dx = (pix - pf.x) + (R)(cxi - cxj) * csf;
-            dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
-            dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
-            r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < hif) && !esc_i && !(pk >> 31)) continue;
-            if (!(r2 < lof) || esc_i || (pk >> 31)) {
-                if (exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) continue;
-            }
-        } else {
-            const double4 qj = D.posh[j];
-            const double dr = exact_dr<DIM>(qi, qj, cut, lo2, hi2);
-            if (dr < 0.0) continue;
-            dx = (R)(qi.x - qj.x);
-            dy = (R)(qi.y - qj.y);
-            dz = DIM == 3 ? (R)(qi.z - qj.z) : (R)0;
-            r2 = dr;
-        }
-        if (need_win) {
-            const uint32_t rcj = D.refcell[j];
-            bool inwin = true;
-            for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
-            if (!inwin) continue;
-        }
-        ++acc_n;
-        R4 vj, thj;
-        R hj, Vj, mj, r, G;
-        if constexpr (MIXED) {
-            // vmf.w = m / rho; one rsq gives r and 1 / r; both kernel
-            // branches are cheap, so select instead of diverging
-            vj = *(const R4*)&D.vmf[j];
-            thj = *(const R4*)&D.thf[j];
-            hj = thj.w;
-            Vj = vj.w;
-            mj = Vj * thj.x;
-            const R rinv = fast_rsq(r2);
-            r = r2 * rinv;
-            const R q = r * inv_h;
-            const R t = (R)2 - q;
-            const R Gn = g1 * ((R)-2 + (R)1.5 * q);
-            const R Gf = g2 * t * t * rinv;
-            G = q < (R)1 ? Gn : Gf;
-        } else {
-            vj = *(const R4*)&D.vm[j];
-            thj = *(const R4*)&D.th[j];
-            hj = (R)D.posh[j].w;
-            Vj = thj.w;
-            mj = vj.w;
-            r = fast_sqrt(r2);
-            const R q = r * inv_h;
-            const R t = (R)2 - q;
-            G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
-        }
+    // physics of one accepted pair, both precisions
+    auto sum_pair = [&](int j, R dx, R dy, R dz, R r2, R r, R G, const R4& vj, const R4& thj, R hj, R Vj,
+                        R mj) {
+        if constexpr (!MIXED) ++acc_n;
         const R gx = G * dx, gy = G * dy, gz = G * dz;
         const R dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = DIM == 3 ? vi.z - vj.z : (R)0;
         const R dvg = dvx * gx + dvy * gy + dvz * gz;
@@ -349,6 +291,78 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             ax += fe * dx;
             ay += fe * dy;
             az += fe * dz;
+        }
+    };
+    auto in_window = [&](int j) {
+        if (!need_win) return true;
+        const uint32_t rcj = D.refcell[j];
+        bool inwin = true;
+        for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
+        return inwin;
+    };
+    if constexpr (MIXED) {
+        auto candidate = [&](int j, const float4& pf, const PairRecF& rj) {
+            const float4& vj = rj.vm;
+            const float4& thj = rj.th;
+            const uint32_t pk = __float_as_uint(pf.w);
+            int cxj, cyj, czj;
+            unpack_cell<DIM>(pk, cxj, cyj, czj);
+            const R dx = (pix - pf.x) + (R)(cxi - cxj) * csf;
+            const R dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
+            const R dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
+            const R r2 = dx * dx + dy * dy + dz * dz;
+            // every candidate runs the pair arithmetic and a rejected one adds
+            // zeros (G = 0): no divergent branch around the sums
+            const bool esc = esc_i || (pk >> 31);
+            bool ok = (r2 < hif) || esc;
+            if (ok && (!(r2 < lof) || esc) && exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) ok = false;
+            if (ok && !in_window(j)) ok = false;
+            // vm.w = m / rho; one rsq gives r and 1 / r; both kernel
+            // branches are cheap, so select instead of diverging
+            const R rinv = fast_rsq(r2);
+            const R r = r2 * rinv;
+            const R q = r * inv_h;
+            const R t = (R)2 - q;
+            const R Gn = g1 * ((R)-2 + (R)1.5 * q);
+            const R Gf = g2 * t * t * rinv;
+            acc_n += ok ? 1 : 0;
+            sum_pair(j, dx, dy, dz, r2, r, ok ? (q < (R)1 ? Gn : Gf) : (R)0, vj, thj, thj.w, vj.w,
+                     vj.w * thj.x);
+        };
+        // software pipeline: the three records of candidate k + 1 are in
+        // flight while candidate k is tested and summed (loaded whether or
+        // not it turns out to be a neighbour)
+        int j = n > 0 ? nl[0] : s;
+        int jn = n > 1 ? nl[P.ldn] : j;
+        float4 pf = D.posf[j];
+        PairRecF rj = D.prf[j];
+        for (int k = 0; k < n; ++k) {
+            const int jnn = k + 2 < n ? nl[(int64_t)(k + 2) * P.ldn] : jn;
+            const float4 pfn = D.posf[jn];
+            const PairRecF rjn = D.prf[jn];
+            candidate(j, pf, rj);
+            j = jn;
+            jn = jnn;
+            pf = pfn;
+            rj = rjn;
+        }
+    } else {
+        int jn = n > 0 ? nl[0] : 0;
+        for (int k = 0; k < n; ++k) {
+            const int j = jn;
+            if (k + 1 < n) jn = nl[(int64_t)(k + 1) * P.ldn];
+            const double4 qj = D.posh[j];
+            const double dr = exact_dr<DIM>(qi, qj, cut, lo2, hi2);
+            if (dr < 0.0) continue;
+            if (!in_window(j)) continue;
+            const R4 vj = *(const R4*)&D.vm[j];
+            const R4 thj = *(const R4*)&D.th[j];
+            const R r = fast_sqrt((R)dr);
+            const R q = r * inv_h;
+            const R t = (R)2 - q;
+            const R G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
+            sum_pair(j, (R)(qi.x - qj.x), (R)(qi.y - qj.y), DIM == 3 ? (R)(qi.z - qj.z) : (R)0, (R)dr, r, G,
+                     vj, thj, (R)qj.w, thj.w, vj.w);
         }
     }
     if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100)) {

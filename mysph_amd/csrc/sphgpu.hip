@@ -37,6 +37,13 @@ using namespace sphgpu;
 
 // ----------------------------------------------------------------- device
 
+// Mixed precision: what the pair loop reads of an accepted neighbour, in one
+// 32-byte record (one cache line).
+struct __align__(32) PairRecF {
+    float4 vm;   // vx, vy, vz, m / rho
+    float4 th;   // rho, p/rho^2, c, h
+};
+
 struct Dev {
     double4* posh;
     double4* vm;
@@ -81,12 +88,12 @@ struct Dev {
     // mixed precision: FP32 copies read by the pair kernels, and FP32 density
     // deviation / derivatives (positions and velocities stay FP64)
     float4* cpos;     // FP32 positions relative to the grid origin, in cell order; w = slot
-    // mixed: separate 16-byte records. The candidate test reads only posf, and
-    // the k-th candidates of neighbouring lanes are neighbouring slots, so its
-    // gathers coalesce best when posf is packed on its own
+    // mixed: the candidate test reads only posf, and the k-th candidates of
+    // neighbouring lanes are neighbouring slots, so its gathers coalesce best
+    // when posf is packed on its own; an accepted neighbour then needs both
+    // halves of its 32-byte pair record, which share one cache line
     float4* posf;     // offset from own cell corner (xyz) + packed cell (w)
-    float4* vmf;      // vx, vy, vz, m / rho
-    float4* thf;      // rho, p/rho^2, c, h
+    PairRecF* prf;    // vm = (vx, vy, vz, m / rho), th = (rho, p/rho^2, c, h)
     float* drho;      // rho - rho0
     float* drho_prev;
     float4* dvf;      // dvdt xyz, d(rho)/dt w
@@ -578,10 +585,12 @@ __device__ __forceinline__ void eos_mixed_vals(const Params& P, const Dev& D, in
     D.p[s] = (double)p;
     float mass = (float)v.w;
     float irho = 1.0f / rho;
-    D.thf[s] = make_float4(rho, p * irho * irho, (float)D.c[s], (float)q.w);
-    // w = m / rho: the pair loop needs the volume of every neighbour and
+    // vm.w = m / rho: the pair loop needs the volume of every neighbour and
     // recovers the mass as V rho (one multiply instead of a reciprocal per pair)
-    D.vmf[s] = make_float4((float)v.x, (float)v.y, (float)v.z, mass * irho);
+    PairRecF rec;
+    rec.vm = make_float4((float)v.x, (float)v.y, (float)v.z, mass * irho);
+    rec.th = make_float4(rho, p * irho * irho, (float)D.c[s], (float)q.w);
+    D.prf[s] = rec;
     // offset from the particle's own cell corner, exact cell size cs: dx of a
     // pair is (off_i - off_j) + (c_i - c_j) cs, accurate to ~1e-7 of a cell
     double xs[3] = {q.x, q.y, q.z};
@@ -1101,8 +1110,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
     int cxi = 0, cyi = 0, czi = 0;
     double4 qi;
     if (MIXED) {
-        vi = *(const R4*)&D.vmf[s];
-        thi = *(const R4*)&D.thf[s];
+        vi = *(const R4*)&D.prf[s].vm;
+        thi = *(const R4*)&D.prf[s].th;
         const float4 pf = D.posf[s];
         pix = pf.x; piy = pf.y; piz = pf.z;
         unpack_cell<DIM>(__float_as_uint(pf.w), cxi, cyi, czi);
@@ -1133,8 +1142,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
         R4 vj, thj;
         R dx, dy, dz, hj, Vj;
         if (MIXED) {
-            vj = *(const R4*)&D.vmf[j];
-            thj = *(const R4*)&D.thf[j];
+            vj = *(const R4*)&D.prf[j].vm;
+            thj = *(const R4*)&D.prf[j].th;
             const float4 pf = D.posf[j];
             int cxj, cyj, czj;
             unpack_cell<DIM>(__float_as_uint(pf.w), cxj, cyj, czj);
@@ -1142,7 +1151,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
             dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
             dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
             hj = thj.w;
-            Vj = vj.w;              // vmf.w = m / rho
+            Vj = vj.w;              // vm.w = m / rho
             vj.w = Vj * thj.x;      // mass
         } else {
             vj = *(const R4*)&D.vm[j];
@@ -1867,7 +1876,7 @@ static int alloc_state(sphgpu_ctx* c) {
     }
     if (c->cfg.kernel_correction) rc |= dalloc(c, &D.kgcA, T);
     if (c->P.mixed) {
-        rc |= dalloc(c, &D.vmf, N); rc |= dalloc(c, &D.thf, N); rc |= dalloc(c, &D.drho, N);
+        rc |= dalloc(c, &D.prf, N); rc |= dalloc(c, &D.drho, N);
         rc |= dalloc(c, &D.posf, N);
         rc |= dalloc(c, &D.drho_prev, N); rc |= dalloc(c, &D.dvf, N); rc |= dalloc(c, &D.def, N);
     }
