@@ -48,10 +48,14 @@ typedef enum sphgpu_status {
 } sphgpu_status;
 
 /* Arithmetic of the pair kernels. Positions, velocities, the neighbour test
- * and the integrator stay FP64 in both modes. */
+ * and the integrator stay FP64 in both modes. MIXED on the fluid set of the
+ * fast path also stores density as an FP32 deviation rho - rho0 and the pair
+ * inputs as FP32 records; outside that set (solids, viscosity, kernel
+ * correction, artificial stress, summation density) the state stays FP64 and
+ * only the pair sums run in FP32. */
 typedef enum sphgpu_precision {
     SPHGPU_FP64 = 0,
-    SPHGPU_MIXED = 1              /* FP32 pair physics, density stored as FP32 rho-rho0 */
+    SPHGPU_MIXED = 1              /* FP32 pair physics */
 } sphgpu_precision;
 
 typedef enum sphgpu_eos {

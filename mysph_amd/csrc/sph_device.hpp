@@ -34,6 +34,7 @@ struct Params {
     int summation_density, normalize_density, viscosity, gravity, gravity_mode;
     int external_force, arti_visc, arti_stress, kgc, has_solid;
     int mixed;          // FP32 pair physics with density stored as rho - rho0 in FP32
+    int pair32;         // general path in mixed precision: FP64 state, FP32 pair arithmetic
     int pair_ext;       // external pair forces can act (wall range > 0 or several fluid types)
     double g, av_alpha, av_beta, av_psi;
     double lj_factor, lj_r0, lj_p1, lj_p2;
@@ -162,71 +163,72 @@ __device__ __forceinline__ double eos_p(const Material* m, double rho, double p_
 }
 
 // Kernel normalisation factor alpha_d (kernel.f90:47-56, :89-92, :124-133).
-template <int DIM>
-__device__ __forceinline__ double kernel_factor(int skf, double h) {
+// R = double, or float for the pair arithmetic of the mixed general path.
+template <int DIM, typename R = double>
+__device__ __forceinline__ R kernel_factor(int skf, R h) {
+    const R pi = (R)kPi;
     if (skf == 1) {
-        if (DIM == 2) return 15.0 / (7.0 * kPi * (h * h));
-        return 3.0 / (2.0 * kPi * (h * h * h));
+        if (DIM == 2) return (R)15.0 / ((R)7.0 * pi * (h * h));
+        return (R)3.0 / ((R)2.0 * pi * (h * h * h));
     } else if (skf == 2) {
-        double sp = sqrt(kPi) * h;
-        return DIM == 2 ? 1.0 / (sp * sp) : 1.0 / (sp * sp * sp);
+        R sp = sqrt(pi) * h;
+        return DIM == 2 ? (R)1.0 / (sp * sp) : (R)1.0 / (sp * sp * sp);
     } else {
-        if (DIM == 2) return 7.0 / (478.0 * kPi * (h * h));
-        return 1.0 / (120.0 * kPi * (h * h * h));
+        if (DIM == 2) return (R)7.0 / ((R)478.0 * pi * (h * h));
+        return (R)1.0 / ((R)120.0 * pi * (h * h * h));
     }
 }
 
 // W and dW/dr-over-r: grad W = G * dx (kernel.f90:8-154).
-template <int DIM>
-__device__ __forceinline__ void kernel_wg(int skf, double r, double h, double factor,
-                                          double* w, double* G) {
-    double q = r / h;
+template <int DIM, typename R = double>
+__device__ __forceinline__ void kernel_wg(int skf, R r, R h, R factor, R* w, R* G) {
+    R q = r / h;
     if (skf == 1) {
-        if (q < 1.0) {
-            *w = factor * (2.0 / 3.0 - q * q + (q * q * q) / 2.0);
-            *G = factor * (-2.0 + 1.5 * q) / (h * h);
-        } else if (q <= 2.0) {
-            double t = 2.0 - q;
-            *w = factor * (1.0 / 6.0) * (t * t * t);
-            *G = factor * (-0.5) * (t * t) / h / r;
+        if (q < (R)1.0) {
+            *w = factor * ((R)(2.0 / 3.0) - q * q + (q * q * q) / (R)2.0);
+            *G = factor * ((R)-2.0 + (R)1.5 * q) / (h * h);
+        } else if (q <= (R)2.0) {
+            R t = (R)2.0 - q;
+            *w = factor * (R)(1.0 / 6.0) * (t * t * t);
+            *G = factor * (R)(-0.5) * (t * t) / h / r;
         } else {
-            *w = 0.0;
-            *G = 0.0;
+            *w = (R)0.0;
+            *G = (R)0.0;
         }
     } else if (skf == 2) {
-        if (q < 3.0) {
+        if (q < (R)3.0) {
             *w = factor * exp(-q * q);
-            *G = *w * (-2.0 / (h * h));
+            *G = *w * ((R)-2.0 / (h * h));
         } else {
-            *w = 0.0;
-            *G = 0.0;
+            *w = (R)0.0;
+            *G = (R)0.0;
         }
     } else {
-        double a = 3.0 - q, b = 2.0 - q, c = 1.0 - q;
-        double a2 = a * a, b2 = b * b, c2 = c * c;
-        if (q < 1.0) {
-            *w = factor * (a2 * a2 * a - 6.0 * (b2 * b2 * b) + 15.0 * (c2 * c2 * c));
-            *G = factor * (-120.0 + 120.0 * q - 50.0 * (q * q)) / (h * h);
-        } else if (q < 2.0) {
-            *w = factor * (a2 * a2 * a - 6.0 * (b2 * b2 * b));
-            *G = factor * (-5.0 * (a2 * a2) + 30.0 * (b2 * b2)) / h / r;
-        } else if (q <= 3.0) {
+        R a = (R)3.0 - q, b = (R)2.0 - q, c = (R)1.0 - q;
+        R a2 = a * a, b2 = b * b, c2 = c * c;
+        if (q < (R)1.0) {
+            *w = factor * (a2 * a2 * a - (R)6.0 * (b2 * b2 * b) + (R)15.0 * (c2 * c2 * c));
+            *G = factor * ((R)-120.0 + (R)120.0 * q - (R)50.0 * (q * q)) / (h * h);
+        } else if (q < (R)2.0) {
+            *w = factor * (a2 * a2 * a - (R)6.0 * (b2 * b2 * b));
+            *G = factor * ((R)-5.0 * (a2 * a2) + (R)30.0 * (b2 * b2)) / h / r;
+        } else if (q <= (R)3.0) {
             *w = factor * (a2 * a2 * a);
-            *G = factor * (-5.0 * (a2 * a2)) / h / r;
+            *G = factor * ((R)-5.0 * (a2 * a2)) / h / r;
         } else {
-            *w = 0.0;
-            *G = 0.0;
+            *w = (R)0.0;
+            *G = (R)0.0;
         }
     }
 }
 
 // Self contribution W(0, h) (density.f90:62).
-template <int DIM>
-__device__ __forceinline__ double kernel_w0(int skf, double h) {
-    double f = kernel_factor<DIM>(skf, h);
-    if (skf == 1) return f * (2.0 / 3.0);
+template <int DIM, typename R = double>
+__device__ __forceinline__ R kernel_w0(int skf, R h) {
+    R f = kernel_factor<DIM, R>(skf, h);
+    if (skf == 1) return f * (R)(2.0 / 3.0);
     if (skf == 2) return f;
-    return f * (243.0 - 6.0 * 32.0 + 15.0);
+    return f * (R)(243.0 - 6.0 * 32.0 + 15.0);
 }
 
 // Reference cell of a position, locate (asymmetricSearch.f90:27-47), with

@@ -739,40 +739,46 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
 }
 
 // Pair geometry of target i and neighbour j: dx = x_i - x_j, r, W, grad W.
-template <int DIM>
+// R = double, or float in the mixed general path (dx formed in FP64 and
+// rounded once; the corrected-gradient solve stays FP64).
+template <int DIM, typename R = double>
 struct PairGeo {
-    double dx[3];
-    double r2, r, w, gw[3];
+    R dx[3];
+    R r2, r, w, gw[3];
 };
 
-template <int DIM>
+template <int DIM, typename R>
 __device__ __forceinline__ void pair_geo(const Params& P, const double4& qi, const double4& qj,
-                                         double factor, const double* A, PairGeo<DIM>& g) {
-    g.dx[0] = qi.x - qj.x;
-    g.dx[1] = qi.y - qj.y;
-    g.dx[2] = DIM == 3 ? qi.z - qj.z : 0.0;
-    double dr = g.dx[0] * g.dx[0] + g.dx[1] * g.dx[1];
+                                         R factor, const double* A, PairGeo<DIM, R>& g) {
+    g.dx[0] = (R)(qi.x - qj.x);
+    g.dx[1] = (R)(qi.y - qj.y);
+    g.dx[2] = DIM == 3 ? (R)(qi.z - qj.z) : (R)0.0;
+    R dr = g.dx[0] * g.dx[0] + g.dx[1] * g.dx[1];
     if (DIM == 3) dr += g.dx[2] * g.dx[2];
     g.r2 = dr;
     g.r = sqrt(dr);
-    double G;
-    kernel_wg<DIM>(P.skf, g.r, qi.w, factor, &g.w, &G);
-    for (int d = 0; d < 3; ++d) g.gw[d] = (d < DIM) ? G * g.dx[d] : 0.0;
-    if (A) mcpe_solve<DIM>(A, g.gw);
+    R G;
+    kernel_wg<DIM, R>(P.skf, g.r, (R)qi.w, factor, &g.w, &G);
+    for (int d = 0; d < 3; ++d) g.gw[d] = (d < DIM) ? G * g.dx[d] : (R)0.0;
+    if (A) {
+        double b[3] = {(double)g.gw[0], (double)g.gw[1], (double)g.gw[2]};
+        mcpe_solve<DIM>(A, b);
+        for (int d = 0; d < 3; ++d) g.gw[d] = (R)b[d];
+    }
 }
 
 // kernelGradientCorrection.f90:20-35: A_i = sum_j V_j (x_j - x_i)(e) dW(d).
-template <int DIM>
+template <int DIM, typename R>
 __global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
     int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= P.n) return;
     int4 mt = D.meta[s];
     if (mt.w >= P.n_real || mt.y == -1) return;
-    double A[DIM * DIM];
-    for (int q = 0; q < DIM * DIM; ++q) A[q] = 0.0;
+    R A[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) A[q] = (R)0.0;
     if (mt.x > 100) {
         double4 qi = D.posh[s];
-        double f = kernel_factor<DIM>(P.skf, qi.w);
+        R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
         int n = min(D.ncnt[s], P.nbr_cap);
         int ai = mt.x < 0 ? -mt.x : mt.x;
         for (int k = 0; k < n; ++k) {
@@ -781,43 +787,43 @@ __global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
             int aj = tj < 0 ? -tj : tj;
             if ((ai - 100) * (aj - 100) < 0) continue;
             double4 qj = D.posh[j];
-            PairGeo<DIM> g;
-            pair_geo<DIM>(P, qi, qj, f, nullptr, g);
-            double V = D.vm[j].w / D.rho[j];
+            PairGeo<DIM, R> g;
+            pair_geo<DIM, R>(P, qi, qj, f, nullptr, g);
+            R V = (R)(D.vm[j].w / D.rho[j]);
             for (int d = 0; d < DIM; ++d)
                 for (int e = 0; e < DIM; ++e) A[d * DIM + e] += (-g.dx[e]) * g.gw[d] * V;
         }
     }
-    for (int q = 0; q < DIM * DIM; ++q) D.kgcA[(int64_t)s * DIM * DIM + q] = A[q];
+    for (int q = 0; q < DIM * DIM; ++q) D.kgcA[(int64_t)s * DIM * DIM + q] = (double)A[q];
 }
 
 // density.f90:10-80 (summation density, optional Shepard normalisation).
-template <int DIM>
+template <int DIM, typename R>
 __global__ __launch_bounds__(kBlock) void k_sumdens(Params P, Dev D) {
     int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= P.n) return;
     int4 mt = D.meta[s];
     if (mt.w >= P.n_real) return;
     double4 qi = D.posh[s];
-    double f = kernel_factor<DIM>(P.skf, qi.w);
-    double w0 = kernel_w0<DIM>(P.skf, qi.w);
-    double mi = D.vm[s].w;
-    double rho = mi * w0;
-    double wi = 0.0;
+    R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
+    R w0 = kernel_w0<DIM, R>(P.skf, (R)qi.w);
+    R mi = (R)D.vm[s].w;
+    R rho = mi * w0;
+    R wi = (R)0.0;
     bool norm = P.normalize_density && D.div[s].x < 1.5;
-    if (norm) wi = mi / D.rho[s] * w0;
+    if (norm) wi = mi / (R)D.rho[s] * w0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     for (int k = 0; k < n; ++k) {
         int j = D.nbr[(int64_t)k * P.ldn + s];
         double4 qj = D.posh[j];
-        PairGeo<DIM> g;
-        pair_geo<DIM>(P, qi, qj, f, nullptr, g);
-        double mj = D.vm[j].w;
+        PairGeo<DIM, R> g;
+        pair_geo<DIM, R>(P, qi, qj, f, nullptr, g);
+        R mj = (R)D.vm[j].w;
         rho += mj * g.w;
-        if (norm) wi += mj / D.rho[j] * g.w;
+        if (norm) wi += mj / (R)D.rho[j] * g.w;
     }
     if (norm) rho = rho / wi;
-    D.rho_new[s] = rho;
+    D.rho_new[s] = (double)rho;
 }
 
 __global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
@@ -830,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
 
 // Strain-rate (and solid rotation-rate) sums, internalForce.f90:48-100, and the
 // fluid viscous heating tdsdt (:108-112). T(d,e) = sum_j V_j (v_j - v_i)_d dW_e.
-template <int DIM>
+template <int DIM, typename R>
 __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
     int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= P.n) return;
@@ -838,23 +844,23 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
     const Material* m = find_mat(P, mt.x);
     double eta = (P.viscosity && m) ? m->eta : 0.0;   // viscosity.f90:14-33
     D.eta[s] = eta;
-    double T[DIM * DIM];
-    for (int q = 0; q < DIM * DIM; ++q) T[q] = 0.0;
+    R T[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) T[q] = (R)0.0;
     bool real = mt.w < P.n_real && mt.y != -1;
     if (real) {
         double4 qi = D.posh[s];
         double4 vi = D.vm[s];
-        double f = kernel_factor<DIM>(P.skf, qi.w);
+        R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
         const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
         int n = min(D.ncnt[s], P.nbr_cap);
         for (int k = 0; k < n; ++k) {
             int j = D.nbr[(int64_t)k * P.ldn + s];
             double4 qj = D.posh[j];
             double4 vj = D.vm[j];
-            PairGeo<DIM> g;
-            pair_geo<DIM>(P, qi, qj, f, A, g);
-            double V = vj.w / D.th[j].x;
-            double dvn[3] = {vj.x - vi.x, vj.y - vi.y, vj.z - vi.z};
+            PairGeo<DIM, R> g;
+            pair_geo<DIM, R>(P, qi, qj, f, A, g);
+            R V = (R)(vj.w / D.th[j].x);
+            R dvn[3] = {(R)(vj.x - vi.x), (R)(vj.y - vi.y), (R)(vj.z - vi.z)};
             for (int d = 0; d < DIM; ++d)
                 for (int e = 0; e < DIM; ++e) T[d * DIM + e] += V * dvn[d] * g.gw[e];
         }
@@ -864,11 +870,11 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
     int64_t o = (int64_t)s * DIM * DIM;
     if (ai <= 100) {
         double tr = 0.0;
-        for (int d = 0; d < DIM; ++d) tr += T[d * DIM + d];
+        for (int d = 0; d < DIM; ++d) tr += (double)T[d * DIM + d];
         double sq = 0.0;
         for (int d = 0; d < DIM; ++d)
             for (int e = 0; e < DIM; ++e) {
-                double E = 2.0 * T[d * DIM + e] - (d == e ? (2.0 / 3.0) * tr : 0.0);
+                double E = 2.0 * (double)T[d * DIM + e] - (d == e ? (2.0 / 3.0) * tr : 0.0);
                 sq += E * E;
                 D.Ep[o + d * DIM + e] = eta * E / (rho * rho);
             }
@@ -876,9 +882,9 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
     } else {
         for (int d = 0; d < DIM; ++d)
             for (int e = 0; e < DIM; ++e) {
-                double E = 0.5 * (T[d * DIM + e] + T[e * DIM + d]);
-                D.Ed[o + d * DIM + e] = E;                                           // edot
-                D.rot[o + d * DIM + e] = 0.5 * (T[d * DIM + e] - T[e * DIM + d]);  // rdot
+                double E = 0.5 * ((double)T[d * DIM + e] + (double)T[e * DIM + d]);
+                D.Ed[o + d * DIM + e] = E;                                                 // edot
+                D.rot[o + d * DIM + e] = 0.5 * ((double)T[d * DIM + e] - (double)T[e * DIM + d]);  // rdot
                 D.Ep[o + d * DIM + e] = eta * E / (rho * rho);
             }
     }
@@ -945,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_solid_point(Params P, Dev D) {
 
 // The force pass: everything single_step adds into Delta (singleStep.f90:139-144)
 // plus continuity density and the divergences.
-template <int DIM>
+template <int DIM, typename R>
 __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= P.n) return;
@@ -957,54 +963,54 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     double4 qi = D.posh[s];
     double4 vi = D.vm[s];
     double4 thi = D.th[s];
-    const double f = kernel_factor<DIM>(P.skf, qi.w);
+    const R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
     const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
-    double Ei[DIM * DIM], Si[DIM * DIM], Ri[4] = {0, 0, 0, 0};
+    R Ei[DIM * DIM], Si[DIM * DIM], Ri[4] = {0, 0, 0, 0};
     const bool visc = P.viscosity && fluid;
     for (int q = 0; q < DIM * DIM; ++q) {
-        Ei[q] = visc ? D.Ep[(int64_t)s * DIM * DIM + q] : 0.0;
-        Si[q] = (!fluid) ? D.Sp[(int64_t)s * DIM * DIM + q] : 0.0;
+        Ei[q] = visc ? (R)D.Ep[(int64_t)s * DIM * DIM + q] : (R)0.0;
+        Si[q] = (!fluid) ? (R)D.Sp[(int64_t)s * DIM * DIM + q] : (R)0.0;
     }
     const bool as_on = DIM == 2 && P.arti_stress && ti >= 100;
-    double wdelta = 1.0;
+    R wdelta = (R)1.0;
     if (P.arti_stress && DIM == 2) {
-        for (int q = 0; q < 4; ++q) Ri[q] = (ti >= 100) ? D.asR[(int64_t)s * 4 + q] : 0.0;
-        double G;
-        kernel_wg<DIM>(P.skf, P.as_delta, qi.w, f, &wdelta, &G);
+        for (int q = 0; q < 4; ++q) Ri[q] = (ti >= 100) ? (R)D.asR[(int64_t)s * 4 + q] : (R)0.0;
+        R G;
+        kernel_wg<DIM, R>(P.skf, (R)P.as_delta, (R)qi.w, f, &wdelta, &G);
     }
-    double acc[3] = {0.0, 0.0, 0.0};
-    double de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0;
+    R acc[3] = {0.0, 0.0, 0.0};
+    R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     for (int k = 0; k < n; ++k) {
         int j = D.nbr[(int64_t)k * P.ldn + s];
         double4 qj = D.posh[j];
         double4 vj = D.vm[j];
         double4 thj = D.th[j];
-        PairGeo<DIM> g;
-        pair_geo<DIM>(P, qi, qj, f, A, g);
-        const double mj = vj.w;
-        double dv[3] = {vi.x - vj.x, vi.y - vj.y, DIM == 3 ? vi.z - vj.z : 0.0};
-        double dvg = dv[0] * g.gw[0] + dv[1] * g.gw[1] + (DIM == 3 ? dv[2] * g.gw[2] : 0.0);
-        double dxg = g.dx[0] * g.gw[0] + g.dx[1] * g.gw[1] + (DIM == 3 ? g.dx[2] * g.gw[2] : 0.0);
+        PairGeo<DIM, R> g;
+        pair_geo<DIM, R>(P, qi, qj, f, A, g);
+        const R mj = (R)vj.w;
+        R dv[3] = {(R)(vi.x - vj.x), (R)(vi.y - vj.y), DIM == 3 ? (R)(vi.z - vj.z) : (R)0.0};
+        R dvg = dv[0] * g.gw[0] + dv[1] * g.gw[1] + (DIM == 3 ? dv[2] * g.gw[2] : (R)0.0);
+        R dxg = g.dx[0] * g.gw[0] + g.dx[1] * g.gw[1] + (DIM == 3 ? g.dx[2] * g.gw[2] : (R)0.0);
         // continuity (density.f90:98-101) and divergences (divergence.f90:17-27)
-        sdiv_v += thj.w * dvg;
-        sdiv_x += thj.w * dxg;
+        sdiv_v += (R)thj.w * dvg;
+        sdiv_x += (R)thj.w * dxg;
         // internal force, SPH = 2 (internalForce.f90:228-274)
-        double aux = thi.y + thj.y;
+        R aux = (R)(thi.y + thj.y);
         if (fluid) {
             for (int d = 0; d < DIM; ++d) {
-                double mv = 0.0;
+                R mv = (R)0.0;
                 if (visc) {
                     const double* Ej = D.Ep + (int64_t)j * DIM * DIM;
-                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + Ej[d * DIM + e]) * g.gw[e];
+                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + (R)Ej[d * DIM + e]) * g.gw[e];
                 }
                 acc[d] += mj * (-aux * g.gw[d] + mv);
             }
         } else {
             const double* Sj = D.Sp + (int64_t)j * DIM * DIM;
             for (int d = 0; d < DIM; ++d) {
-                double sv = 0.0;
-                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + Sj[d * DIM + e]) * g.gw[e];
+                R sv = (R)0.0;
+                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + (R)Sj[d * DIM + e]) * g.gw[e];
                 acc[d] += mj * sv;
             }
         }
@@ -1012,68 +1018,69 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         // external pair forces (externalForce.f90:103-174)
         if (P.external_force) {
             const int tj = D.meta[j].x;
-            if (ti > 0 && tj < 0 && g.r < P.lj_r0) {
-                double rr = P.lj_r0 / g.r;
-                double fl = P.lj_factor * (pow(rr, P.lj_p1) - pow(rr, P.lj_p2)) / (g.r * g.r);
+            if (ti > 0 && tj < 0 && (double)g.r < P.lj_r0) {
+                R rr = (R)P.lj_r0 / g.r;
+                R fl = (R)P.lj_factor * (pow(rr, (R)P.lj_p1) - pow(rr, (R)P.lj_p2)) / (g.r * g.r);
                 for (int d = 0; d < DIM; ++d) acc[d] += fl * g.dx[d];
             }
             if (ti != tj && ti > 0 && tj > 0) {
                 if (ti < 100 && tj < 100) {
-                    double pe = (qi.w + qj.w) / (2 * g.r);
-                    if (pe >= 1) {
-                        double fp = P.pen_factor * (pow(pe, P.pen_n1) - pow(pe, P.pen_n2)) / (g.r * g.r);
+                    R pe = (R)(qi.w + qj.w) / ((R)2 * g.r);
+                    if (pe >= (R)1) {
+                        R fp = (R)P.pen_factor * (pow(pe, (R)P.pen_n1) - pow(pe, (R)P.pen_n2)) / (g.r * g.r);
                         for (int d = 0; d < DIM; ++d) acc[d] += fp * g.dx[d];
                     }
                 } else if (!(ti > 100 && tj > 100)) {
-                    double et = g.r / 0.75 / ((qi.w + qj.w) * 0.5);
-                    double chi = 1 - g.r / P.couple_delta;
-                    if (chi < 0 || chi > 1) chi = 0;
-                    double fc = 0;
-                    if (et <= 0 || et >= 2) fc = 0;
-                    else if (et <= 2.0 / 3) fc = 2.0 / 3;
-                    else if (et <= 1) fc = 2 * et - 1.5 * et * et;
-                    else if (et < 2) fc = 0.5 * ((2 - et) * (2 - et));
-                    double fcp = (double)0.01f * (thi.z * thi.z) * chi * fc / (g.r * g.r);
+                    R et = g.r / (R)0.75 / ((R)(qi.w + qj.w) * (R)0.5);
+                    R chi = (R)1 - g.r / (R)P.couple_delta;
+                    if (chi < (R)0 || chi > (R)1) chi = (R)0;
+                    R fc = (R)0;
+                    if (et <= (R)0 || et >= (R)2) fc = (R)0;
+                    else if (et <= (R)(2.0 / 3)) fc = (R)(2.0 / 3);
+                    else if (et <= (R)1) fc = (R)2 * et - (R)1.5 * et * et;
+                    else if (et < (R)2) fc = (R)0.5 * (((R)2 - et) * ((R)2 - et));
+                    R fcp = (R)((double)0.01f * (thi.z * thi.z)) * chi * fc / (g.r * g.r);
                     for (int d = 0; d < DIM; ++d) acc[d] += fcp * g.dx[d];
                 }
             }
         }
         // Monaghan artificial viscosity (artificialViscosity.f90:80-110)
         if (P.arti_visc) {
-            double xv = g.dx[0] * dv[0] + g.dx[1] * dv[1] + (DIM == 3 ? g.dx[2] * dv[2] : 0.0);
-            if (xv < 0.0) {
-                double hij = 0.5 * (qi.w + qj.w);
-                double rhoij = 0.5 * (thi.x + thj.x);
-                double cij = 0.5 * (thi.z + thj.z);
-                double ph = P.av_psi * hij;
-                double phi = hij * xv / (g.r2 + ph * ph);
-                double PI_ij = (P.av_beta * phi - P.av_alpha * cij) * phi / rhoij;
+            R xv = g.dx[0] * dv[0] + g.dx[1] * dv[1] + (DIM == 3 ? g.dx[2] * dv[2] : (R)0.0);
+            if (xv < (R)0.0) {
+                R hij = (R)(0.5 * (qi.w + qj.w));
+                R rhoij = (R)(0.5 * (thi.x + thj.x));
+                R cij = (R)(0.5 * (thi.z + thj.z));
+                R ph = (R)P.av_psi * hij;
+                R phi = hij * xv / (g.r2 + ph * ph);
+                R PI_ij = ((R)P.av_beta * phi - (R)P.av_alpha * cij) * phi / rhoij;
                 for (int d = 0; d < DIM; ++d) acc[d] -= mj * PI_ij * g.gw[d];
-                de_av += 0.5 * mj * PI_ij * dvg;
+                de_av += (R)0.5 * mj * PI_ij * dvg;
             }
         }
         // artificial stress (artificialTerm.f90:72-93)
         if (as_on) {
             const int tj = D.meta[j].x;
             if (tj >= 100) {
-                double F = pow(g.w / wdelta, P.as_exponent);
+                R F = pow(g.w / wdelta, (R)P.as_exponent);
                 const double* Rj = D.asR + (int64_t)j * 4;
                 for (int d = 0; d < 2; ++d) {
-                    double sv = 0.0;
-                    for (int e = 0; e < 2; ++e) sv += (Ri[d * 2 + e] + Rj[d * 2 + e]) * g.gw[e];
+                    R sv = (R)0.0;
+                    for (int e = 0; e < 2; ++e) sv += (Ri[d * 2 + e] + (R)Rj[d * 2 + e]) * g.gw[e];
                     acc[d] += mj * F * sv;
                 }
             }
         }
     }
+    double ac[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
     if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100))
-        acc[DIM - 1] += P.g;
-    if (mt.z == 1) acc[0] = acc[1] = acc[2] = 0.0;   // fixedBoundary, boundaryCondition.f90:32-45
-    double drho = P.summation_density ? 0.0 : thi.x * sdiv_v;
-    D.dv[s] = make_double4(acc[0], acc[1], acc[2], drho);
+        ac[DIM - 1] += P.g;
+    if (mt.z == 1) ac[0] = ac[1] = ac[2] = 0.0;   // fixedBoundary, boundaryCondition.f90:32-45
+    double drho = P.summation_density ? 0.0 : thi.x * (double)sdiv_v;
+    D.dv[s] = make_double4(ac[0], ac[1], ac[2], drho);
     double tds = (P.viscosity || !fluid) ? D.tds[s] : 0.0;
-    D.de[s] = tds + 0.5 * de_in + de_av;
-    D.div[s] = make_double2(-sdiv_x, -sdiv_v);
+    D.de[s] = tds + 0.5 * (double)de_in + (double)de_av;
+    D.div[s] = make_double2(-(double)sdiv_x, -(double)sdiv_v);
 }
 
 // Fluid force pass for the dam-break physics set: cubic-spline kernel,
@@ -1518,6 +1525,7 @@ static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_
     P.as_lambda = g.as_lambda; P.as_delta = g.as_delta; P.as_exponent = g.as_exponent;
     P.dt = g.dt;
     P.mixed = g.precision == SPHGPU_MIXED;
+    P.pair32 = 0;
 }
 
 // The fused fluid force kernel covers the dam-break physics set; anything
@@ -1793,13 +1801,20 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, mode);
     if (!fast) {
-        if (P.kgc) hipLaunchKernelGGL(k_kgc<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        // pair32: the pair sums of the general path in FP32 (mixed precision)
+        if (P.kgc) {
+            if (P.pair32) hipLaunchKernelGGL((k_kgc<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+            else hipLaunchKernelGGL((k_kgc<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        }
         if (P.summation_density) {
-            hipLaunchKernelGGL(k_sumdens<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+            if (P.pair32) hipLaunchKernelGGL((k_sumdens<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+            else hipLaunchKernelGGL((k_sumdens<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
             hipLaunchKernelGGL(k_apply_sumdens, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
         }
-        if (P.viscosity || P.has_solid)
-            hipLaunchKernelGGL(k_strain<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        if (P.viscosity || P.has_solid) {
+            if (P.pair32) hipLaunchKernelGGL((k_strain<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+            else hipLaunchKernelGGL((k_strain<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        }
         if (P.has_solid)
             hipLaunchKernelGGL(k_solid_point<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     }
@@ -1812,8 +1827,10 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
         hipLaunchKernelGGL((k_force_fluid<DIM, false, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     else if (fast)
         hipLaunchKernelGGL((k_force_fluid<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (P.pair32)
+        hipLaunchKernelGGL((k_force<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     else
-        hipLaunchKernelGGL(k_force<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     if (c->time_force) hipEventRecord(c->ev[3], c->stream);
 }
 
@@ -2017,9 +2034,10 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     }
     hr.fast = fast_path_ok(cfg, P, force_general);
     if (P.mixed && !hr.fast) {
-        why = "mixed precision covers the fluid set (cubic spline, continuity density, "
-              "no physical viscosity, solids, kernel correction or artificial stress)";
-        return SPHGPU_E_BAD_CONFIG;
+        // outside the fluid set, mixed precision keeps the FP64 state of the
+        // general path and runs its pair sums in FP32
+        P.mixed = 0;
+        P.pair32 = 1;
     }
     if (cfg.kernel_correction)
         for (int64_t i = 0; i < n_real; ++i)
