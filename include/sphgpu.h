@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SPHGPU_ABI_VERSION 1
+#define SPHGPU_ABI_VERSION 2
 #define SPHGPU_MAX_MATERIALS 8
 
 typedef enum sphgpu_status {
@@ -109,6 +109,14 @@ typedef struct sphgpu_config {
     double as_lambda, as_delta, as_exponent;     /* artificialTerm.f90:17-32 */
     double dt;                                   /* Config%delta_t */
     sphgpu_material mat[SPHGPU_MAX_MATERIALS];
+    /* No counterpart in the reference. 1: the physical-viscosity pair term
+     * of a fluid particle (internalForce.f90:236-244) skips solid
+     * neighbours, whose eta is their shear modulus G (viscosity.f90:24).
+     * With 0 (the reference), a fluid next to an elastic solid takes
+     * G * strain rate as a viscous stress, which no explicit time step can
+     * integrate stably (diffusion number G dt / (rho h^2) >> 1). */
+    int32_t fluid_visc_skip_solid;
+    int32_t reserved;
 } sphgpu_config;
 
 /* Field selectors for sphgpu_download / sphgpu_upload_field. */

@@ -277,6 +277,93 @@ def dam_break_3d_fluid_count(dx: float) -> int:
     return int(round(0.6 / dx)) ** 2 * int(round(1.2 / dx))
 
 
+def dam_break_gate_3d(dx_f: float = 0.0016, dx_s: float = 0.002, width: float = 0.6,
+                      column=(0.1, 0.14), tank=(0.2, 0.15), gate=(0.005, 0.1), layers: int = 3,
+                      h_over_dx: float = 1.2, jitter: float = 0.01, precision: int = C.FP64,
+                      alpha: float = 0.1, seed: int = SEED_JITTER) -> dict:
+    """3D dam break with an elastic gate: the reference's 2D "db_gate"
+    layout (input.f90:1090-1154, dummyParticles.f90:661-765) extruded over
+    ``width`` in y, at two resolutions.
+
+    Water column x in [0, column[0]], z in [0, column[1]] on a dx_f lattice
+    (jittered as the other benchmark cases). Gate: a type-102 plate of
+    int(gate[0] / dx_s) + 1 layers at x = column[0] + (i + 1/2) dx_s, rows
+    z = (j + 3/2) dx_s for j < int(gate[1] / dx_s) + 1 (a gap above the floor,
+    as input.f90:1138 leaves one), the top row clamped (Boundary = 1, as
+    input.f90:1147-1149). Fixed boundary (type -2, ``layers`` deep, dx_f):
+    floor, the four tank walls to tank[1], and the wall above the gate from
+    one dx_f over its clamped row to tank[1] (the reference's "Middle Wall").
+    h = h_over_dx * dx per phase: the smoothing length varies by particle.
+    The defaults give 2,046,000 fluid and 45,900 solid particles."""
+    dim = 3
+    L, Hw = tank
+    Wc, Hc = column
+    h_f, h_s = h_over_dx * dx_f, h_over_dx * dx_s
+    cfg = C.gate_config(dim, Hc, min(h_f, h_s), alpha=alpha, couple_delta=dx_f, precision=precision)
+    B = cfg.material(2).K
+    c0 = C.sound_speed(cfg, 2)
+    cs = C.sound_speed(cfg, 102)
+    g = 9.81
+    # water
+    n = [int(round(Wc / dx_f)), int(round(width / dx_f)), int(round(Hc / dx_f))]
+    grids = np.meshgrid(*[np.arange(k) for k in n], indexing="ij")
+    idx = np.stack([gr.ravel() for gr in grids], axis=1)
+    nf = idx.shape[0]
+    xf = (idx + 0.5) * dx_f
+    if jitter:
+        xf = xf + jitter * dx_f * jitter_unit(np.arange(nf), dim, seed)
+    fl = _empty(nf, dim)
+    fl["x"][:] = xf
+    fl["mass"][:] = 1000.0 * dx_f ** 3
+    fl["p"][:] = 1000.0 * g * np.maximum(Hc - xf[:, 2], 0.0)
+    fl["rho"][:] = _tait_inverse(fl["p"], B)
+    fl["c"][:] = c0
+    fl["h"][:] = h_f
+    fl["type"][:] = 2
+    # gate
+    ns = [int(gate[0] / dx_s) + 1, int(round(width / dx_s)), int(gate[1] / dx_s) + 1]
+    grids = np.meshgrid(*[np.arange(k) for k in ns], indexing="ij")
+    sidx = np.stack([gr.ravel() for gr in grids], axis=1)
+    xs = np.empty((len(sidx), 3))
+    xs[:, 0] = Wc + (sidx[:, 0] + 0.5) * dx_s
+    xs[:, 1] = (sidx[:, 1] + 0.5) * dx_s
+    xs[:, 2] = (sidx[:, 2] + 1.5) * dx_s
+    so = _empty(len(sidx), dim)
+    so["x"][:] = xs
+    so["rho"][:] = 1100.0
+    so["mass"][:] = 1100.0 * dx_s ** 3
+    so["c"][:] = cs
+    so["h"][:] = h_s
+    so["type"][:] = 102
+    so["boundary"][:] = (sidx[:, 2] == ns[2] - 1).astype(np.int32)
+    gate_x1 = xs[:, 0].max() + 0.5 * dx_s
+    gate_z1 = xs[:, 2].max()
+    # fixed boundary on the dx_f lattice
+    nt = [int(round(L / dx_f)), int(round(width / dx_f)), int(round(Hw / dx_f))]
+    lo = [-layers, -layers, -layers]
+    hi = [nt[0] + layers, nt[1] + layers, nt[2]]
+    grids = np.meshgrid(*[np.arange(lo[d], hi[d]) for d in range(dim)], indexing="ij")
+    bidx = np.stack([gr.ravel() for gr in grids], axis=1)
+    xb = (bidx + 0.5) * dx_f
+    inside = (bidx[:, 0] >= 0) & (bidx[:, 0] < nt[0]) & (bidx[:, 1] >= 0) & (bidx[:, 1] < nt[1]) & \
+             (bidx[:, 2] >= 0)
+    middle = inside & (xb[:, 0] > Wc) & (xb[:, 0] < gate_x1) & (xb[:, 2] > gate_z1 + dx_f)
+    keep = ~inside | middle
+    xb = xb[keep]
+    bd = _empty(len(xb), dim)
+    bd["x"][:] = xb
+    bd["mass"][:] = 1000.0 * dx_f ** 3
+    bd["p"][:] = 1000.0 * g * np.maximum(Hc - xb[:, 2], 0.0)
+    bd["rho"][:] = _tait_inverse(bd["p"], B)
+    bd["c"][:] = c0
+    bd["h"][:] = h_f
+    bd["type"][:] = -2
+    case = _concat([fl, so, bd], dim)
+    case.update(n_real=nf + len(xs), n_dummy=len(xb), dim=dim, config=cfg,
+                name=f"dam_break_gate_3d_dx{dx_f:g}_{dx_s:g}", n_fluid=nf, n_solid=len(xs))
+    return case
+
+
 def elastic_beam(dx: float = 0.0005, length: float = 0.2, thickness: float = 0.02,
                  E: float = 2e9, nu: float = 0.3975, rho0: float = 1000.0,
                  amplitude: float = 0.01, clamp_columns: int = 3,

@@ -97,6 +97,8 @@ class Config(ctypes.Structure):
         ("as_exponent", ctypes.c_double),
         ("dt", ctypes.c_double),
         ("mat", Material * MAX_MATERIALS),
+        ("fluid_visc_skip_solid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def add_material(self, type_: int, eos: int, rho0: float, K: float,
@@ -204,6 +206,45 @@ def dam_break_config(dim: int, column_height: float, h: float, alpha: float = 0.
     c.dt = cfl * h / (c0 * (1.0 + 1.2 * alpha))
     c.precision = precision
     c.add_material(2, EOS_TAIT, 1000.0, tait_B(c0), 7.0, eta=f32(1e-3))
+    return c
+
+
+def gate_config(dim: int, column_height: float, h_min: float, E: float = 12e6, nu: float = 0.4,
+                rho_s: float = 1100.0, couple_delta: float = 0.001, alpha: float = 0.1,
+                beta: float = 0.0, psi: float = 0.1, cfl: float = 0.2,
+                precision: int = FP64) -> Config:
+    """Dam break with an elastic gate: the benchmark water (Tait,
+    c0 = 10 sqrt(2 g H)) and a Hookean solid of type 102 with K = E / (3 (1 -
+    2 nu)) and G = E / (2 (1 + nu)) (E = 12 MPa, nu = 0.4 give the
+    reference's type 102, eos.f90:219-226 and viscosity.f90:24), the
+    fluid-solid coupling force within ``couple_delta`` (externalForce.f90:
+    142-170; 0.001 for nick "db_gate", :93-95), viscosity on, which the
+    solid's strain-rate pass needs (internalForce.f90:48-100;
+    data/damBreakwithElasticGate/sph.toml), and fixed dt =
+    cfl * h_min / (c_max (1 + 1.2 alpha)) over both sound speeds.
+
+    fluid_visc_skip_solid is on: with the reference's pair term a fluid
+    next to the gate takes eta_j = G (4.3 MPa) times the solid's strain
+    rate as a viscous stress, and the run diverges within a few steps at
+    any practical dt."""
+    c = base_config(dim)
+    c.summation_density = 0
+    c.viscosity = 1
+    c.gravity = 1
+    c.external_force = 1
+    c.artificial_viscosity = 1
+    c.av_alpha, c.av_beta, c.av_psi = alpha, beta, psi
+    c.lj_r0 = 0.0
+    c.couple_delta = couple_delta
+    c0 = 10.0 * math.sqrt(2.0 * 9.81 * column_height)
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    G = E / (2.0 * (1.0 + nu))
+    cs = math.sqrt(K / rho_s)
+    c.dt = cfl * h_min / (max(c0, cs) * (1.0 + 1.2 * alpha))
+    c.precision = precision
+    c.add_material(2, EOS_TAIT, 1000.0, tait_B(c0), 7.0, eta=f32(1e-3))
+    c.add_material(102, EOS_LINEAR, rho_s, K / rho_s, 1.0, eta=G)
+    c.fluid_visc_skip_solid = 1
     return c
 
 

@@ -35,6 +35,7 @@ struct Params {
     int external_force, arti_visc, arti_stress, kgc, has_solid;
     int mixed;          // FP32 pair physics with density stored as rho - rho0 in FP32
     int pair32;         // general path in mixed precision: FP64 state, FP32 pair arithmetic
+    int visc_skip_solid;  // a fluid's viscous pair term skips solid neighbours
     int pair_ext;       // external pair forces can act (wall range > 0 or several fluid types)
     double g, av_alpha, av_beta, av_psi;
     double lj_factor, lj_r0, lj_p1, lj_p2;

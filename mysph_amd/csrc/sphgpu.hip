@@ -998,9 +998,11 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         // internal force, SPH = 2 (internalForce.f90:228-274)
         R aux = (R)(thi.y + thj.y);
         if (fluid) {
+            const int tvj = D.meta[j].x;
+            const bool vj_on = visc && !(P.visc_skip_solid && (tvj > 100 || tvj < -100));
             for (int d = 0; d < DIM; ++d) {
                 R mv = (R)0.0;
-                if (visc) {
+                if (vj_on) {
                     const double* Ej = D.Ep + (int64_t)j * DIM * DIM;
                     for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + (R)Ej[d * DIM + e]) * g.gw[e];
                 }
@@ -1526,6 +1528,7 @@ static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_
     P.dt = g.dt;
     P.mixed = g.precision == SPHGPU_MIXED;
     P.pair32 = 0;
+    P.visc_skip_solid = g.fluid_visc_skip_solid != 0;
 }
 
 // The fused fluid force kernel covers the dam-break physics set; anything

@@ -68,6 +68,8 @@ module sphgpu_m
         real(c_double) :: as_lambda, as_delta, as_exponent
         real(c_double) :: dt
         type(sphgpu_material) :: mat(SPHGPU_MAX_MATERIALS)
+        integer(c_int32_t) :: fluid_visc_skip_solid
+        integer(c_int32_t) :: reserved
     end type sphgpu_config
 
     public :: sphgpu_abi_version, sphgpu_create, sphgpu_upload, sphgpu_derivatives, &
@@ -234,6 +236,8 @@ contains
         c%couple_delta = f32(0.001_8)
         c%as_lambda = f32(0.2_8); c%as_delta = 0; c%as_exponent = 4
         c%dt = 5e-4_8
+        c%fluid_visc_skip_solid = 0
+        c%reserved = 0
         do k = 1, SPHGPU_MAX_MATERIALS
             c%mat(k) = sphgpu_material(0, 0, 0.0_8, 0.0_8, 1.0_8, 0.0_8, 0.0_8)
         end do

@@ -103,7 +103,7 @@ def load_library(path: str = LIB_PATH):
     L.sphgpu_batch_last_error.restype = ctypes.c_char_p
     L.sphgpu_batch_destroy.argtypes = [vp]
     L.sphgpu_batch_destroy.restype = None
-    if L.sphgpu_abi_version() != 1:
+    if L.sphgpu_abi_version() != 2:
         raise RuntimeError("libsphgpu ABI version mismatch")
     _lib = L
     return L

@@ -622,9 +622,10 @@ static void or_internal_force(const sphgpu_config* cfg, or_state* s, const or_pa
                 const double* Ej = edot + (int64_t)j * dim * dim;
                 double rj2 = s->rho[j] * s->rho[j];
                 double aux = s->p[i] / ri2 + s->p[j] / rj2;
+                const int skip = cfg->fluid_visc_skip_solid && abs(s->type[j]) > 100;
                 for (int d = 0; d < dim; d++) {
                     double mv = 0;
-                    for (int e = 0; e < dim; e++)
+                    for (int e = 0; e < dim && !skip; e++)
                         mv = mv + (s->eta[i] * Ei[d * dim + e] / ri2 + s->eta[j] * Ej[d * dim + e] / rj2) * dw[e];
                     acc[d] = acc[d] + s->mass[j] * (-aux * dw[d] + mv);
                 }

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_entry_point():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(sphgpu.EXPORTED)
-    assert lib.sphgpu_abi_version() == 1
+    assert lib.sphgpu_abi_version() == 2
 
 
 def test_config_struct_layout_matches_c():

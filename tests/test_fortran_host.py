@@ -25,8 +25,8 @@ def built():
 def test_abi_matches_c_layout(tmp_path):
     r = run_host(tmp_path, "--abi")
     words = r.stdout.split()
-    assert words[words.index("abi") + 1] == "1"
-    assert int(words[words.index("config_bytes") + 1]) == ctypes.sizeof(C.Config) == 568
+    assert words[words.index("abi") + 1] == "2"
+    assert int(words[words.index("config_bytes") + 1]) == ctypes.sizeof(C.Config) == 576
 
 
 @pytest.mark.parametrize("dim,dx", [(2, 0.04), (3, 0.03), (2, 0.0137)])
