@@ -116,7 +116,13 @@ typedef struct sphgpu_config {
      * G * strain rate as a viscous stress, which no explicit time step can
      * integrate stably (diffusion number G dt / (rho h^2) >> 1). */
     int32_t fluid_visc_skip_solid;
-    int32_t reserved;
+    /* Config%sle, smoothing-length evolution after the force pass
+     * (singleStep.f90:136, hsml.f90:8-38), real particles only: 0 keep h,
+     * 1 h = 2 (m/rho)^(1/dim), 2 h += dt (h/dim) divergenceVelocity. The
+     * search grid and the neighbour capacity are planned for h up to 1.5x
+     * the larger of its uploaded value and (sle 1) its first update; beyond
+     * that a step fails with SPHGPU_E_OUT_OF_RANGE (upload again). */
+    int32_t sle;
 } sphgpu_config;
 
 /* Field selectors for sphgpu_download / sphgpu_upload_field. */
@@ -207,8 +213,28 @@ int sphgpu_timing(sphgpu_ctx* ctx, double* advance_ms, double* force_kernel_ms,
 /* Number of the step the next sphgpu_advance performs (1 after upload). */
 int64_t sphgpu_step(sphgpu_ctx* ctx);
 
+/* Exact checkpoint of a context: every device array (state, integrator
+ * history, slot order, cell tables, candidate list, rebuild gate) and the
+ * step state, so that a context restored from it continues bit for bit.
+ * The reference's nearest counterpart is its particle file
+ * (read_initial_file / write_initial_file, input.f90:22-92, through
+ * SPH.f90:56-93), which holds rounded fields and no integrator history.
+ * The blob is private to this library version; restore refuses another's.
+ * sphgpu_checkpoint_size gives the bytes sphgpu_checkpoint_save writes;
+ * sphgpu_checkpoint_restore creates a new context (failure message via
+ * sphgpu_last_error(NULL), as for sphgpu_create). */
+int sphgpu_checkpoint_size(sphgpu_ctx* ctx, int64_t* bytes);
+int sphgpu_checkpoint_save(sphgpu_ctx* ctx, void* buf, int64_t bytes);
+int sphgpu_checkpoint_restore(const void* buf, int64_t bytes, sphgpu_ctx** out);
+
+/* What a context holds: its config (cfg_out, may be NULL) and counts3 =
+ * (n_real, n_dummy, capacity). For a caller that gets a context from
+ * sphgpu_checkpoint_restore. No reference counterpart. */
+int sphgpu_context_info(sphgpu_ctx* ctx, sphgpu_config* cfg_out, int64_t* counts3);
+
 /* Message of the last failed call on ctx; with ctx == NULL, the message of
- * the last failed sphgpu_create on the calling thread. */
+ * the last failed sphgpu_create or sphgpu_checkpoint_restore on the calling
+ * thread. */
 const char* sphgpu_last_error(sphgpu_ctx* ctx);
 
 void sphgpu_destroy(sphgpu_ctx* ctx);

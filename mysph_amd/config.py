@@ -98,7 +98,7 @@ class Config(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("mat", Material * MAX_MATERIALS),
         ("fluid_visc_skip_solid", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sle", ctypes.c_int32),
     ]
 
     def add_material(self, type_: int, eos: int, rho0: float, K: float,

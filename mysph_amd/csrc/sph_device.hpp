@@ -20,6 +20,7 @@ constexpr int kNumXcd = 8;
 constexpr int kErrOutOfRange = 1;
 constexpr int kErrNonFinite = 2;
 constexpr int kErrPairCap = 4;
+constexpr int kErrHRange = 8;    // an evolving smoothing length outgrew the planned search range
 
 struct Material {
     int32_t type, eos;
@@ -36,6 +37,7 @@ struct Params {
     int mixed;          // FP32 pair physics with density stored as rho - rho0 in FP32
     int pair32;         // general path in mixed precision: FP64 state, FP32 pair arithmetic
     int visc_skip_solid;  // a fluid's viscous pair term skips solid neighbours
+    int sle;              // smoothing-length evolution (0 none, 1 from density, 2 from div v)
     int pair_ext;       // external pair forces can act (wall range > 0 or several fluid types)
     double g, av_alpha, av_beta, av_psi;
     double lj_factor, lj_r0, lj_p1, lj_p2;
@@ -58,7 +60,8 @@ struct Params {
     double disp_thr2;   // (skin/2)^2, slightly reduced
     double disp_half;   // skin/2, slightly reduced
     int verlet_cap;
-    double hmax;        // largest smoothing length
+    double hmax;        // largest smoothing length (the planned limit when h evolves)
+    double hmax_upload; // largest uploaded smoothing length
     // where this run lives: a single-run context has s0 = 0, c0 = 0; a run of
     // a batch owns slots [s0, s0 + n) and cells [c0, c0 + ncell] of the
     // shared arena, and its own grid, gate and error words

@@ -949,6 +949,29 @@ __global__ __launch_bounds__(kBlock) void k_solid_point(Params P, Dev D) {
     }
 }
 
+// Smoothing-length evolution after the force pass (singleStep.f90:136,
+// hsml.f90:8-38), real particles: sle 1 h = 2 (m/rho)^(1/dim) with the
+// density of this step; sle 2 h += dt (h/dim) divergenceVelocity, undone if
+// it would reach h <= 0. h beyond the planned range fails the step.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_hsml(Params P, Dev D) {
+    int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    if (D.meta[s].w >= P.n_real) return;
+    double4 q = D.posh[s];
+    double h = q.w;
+    if (P.sle == 1) {
+        h = 2.0 * pow(D.vm[s].w / D.rho[s], 1.0 / DIM);
+    } else if (P.sle == 2) {
+        const double dh = (q.w / DIM) * D.div[s].y;
+        h = q.w + P.dt * dh;
+        if (h <= 0) h = h - P.dt * dh;
+    }
+    if (!(h <= P.hmax)) atomicOr(D.err, kErrHRange);
+    q.w = h;
+    D.posh[s] = q;
+}
+
 // The force pass: everything single_step adds into Delta (singleStep.f90:139-144)
 // plus continuity density and the divergences.
 template <int DIM, typename R>
@@ -1372,6 +1395,7 @@ struct sphgpu_ctx {
     hipStream_t stream = nullptr;
     Dev D{};
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;   // size of each allocation (checkpoints copy them whole)
     int64_t ncell_alloc = 0;
     int64_t nbr_alloc = 0;
     int64_t vnbr_alloc = 0;
@@ -1410,6 +1434,7 @@ static int dalloc(sphgpu_ctx* c, T** p, size_t count) {
     HIPCHK(c, hipMalloc(&q, bytes));
     HIPCHK(c, hipMemsetAsync(q, 0, bytes, c->stream));
     c->allocs.push_back(q);
+    c->alloc_bytes.push_back(bytes);
     *p = static_cast<T*>(q);
     return 0;
 }
@@ -1417,6 +1442,7 @@ static int dalloc(sphgpu_ctx* c, T** p, size_t count) {
 static void free_all(sphgpu_ctx* c) {
     for (void* q : c->allocs) hipFree(q);
     c->allocs.clear();
+    c->alloc_bytes.clear();
     c->D = Dev{};
     c->ncell_alloc = 0;
     c->nbr_alloc = 0;
@@ -1435,6 +1461,7 @@ static int validate_config(const sphgpu_config* cfg, std::string* why) {
         *why = "n_materials out of range";
         return SPHGPU_E_BAD_CONFIG;
     }
+    if (cfg->sle < 0 || cfg->sle > 2) { *why = "sle must be 0, 1 or 2"; return SPHGPU_E_BAD_CONFIG; }
     if (cfg->precision != SPHGPU_FP64 && cfg->precision != SPHGPU_MIXED) {
         *why = "unknown precision";
         return SPHGPU_E_BAD_CONFIG;
@@ -1529,6 +1556,7 @@ static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_
     P.mixed = g.precision == SPHGPU_MIXED;
     P.pair32 = 0;
     P.visc_skip_solid = g.fluid_visc_skip_solid != 0;
+    P.sle = g.sle;
 }
 
 // The fused fluid force kernel covers the dam-break physics set; anything
@@ -1536,7 +1564,7 @@ static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_
 // artificial stress, Gauss/quintic kernels) takes the general kernels.
 static bool fast_path_ok(const sphgpu_config& g, const Params& P, bool force_general) {
     return g.skf == 1 && !g.viscosity && !P.has_solid && !g.kernel_correction &&
-           !g.summation_density && !g.artificial_stress && !force_general;
+           !g.summation_density && !g.artificial_stress && g.sle == 0 && !force_general;
 }
 
 static bool fluid_fast_path(const sphgpu_ctx* c) { return fast_path_ok(c->cfg, c->P, c->force_general); }
@@ -1560,8 +1588,10 @@ static Tuning env_tuning() {
     return t;
 }
 
+constexpr double kSleHeadroom = 1.5;
+
 static void plan_grid_params(Params& P, int dim, bool verlet, int64_t n, const double* x,
-                             const double* h, Tuning tune) {
+                             const double* h, Tuning tune, double h_next_max = 0.0) {
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, hmax = 0;
     for (int d = 0; d < dim; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
     for (int64_t i = 0; i < n; ++i) {
@@ -1572,6 +1602,11 @@ static void plan_grid_params(Params& P, int dim, bool verlet, int64_t n, const d
         }
         hmax = std::max(hmax, h[i]);
     }
+    // an evolving h gets headroom: the grid, the window test and the
+    // neighbour capacity hold for h up to kSleHeadroom times the larger of
+    // its upload value and its first update (sle 1 resets h from density)
+    P.hmax_upload = hmax;
+    if (P.sle) hmax = std::max(hmax, h_next_max) * kSleHeadroom;
     P.win = tune.win;
     const double cut_max = hmax * P.scale_k;
     P.skin = verlet ? tune.skin_frac * cut_max : 0.0;
@@ -1835,6 +1870,7 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
     else
         hipLaunchKernelGGL((k_force<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     if (c->time_force) hipEventRecord(c->ev[3], c->stream);
+    if (P.sle) hipLaunchKernelGGL(k_hsml<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
 }
 
 template <int DIM>
@@ -1857,6 +1893,9 @@ static int check_err(sphgpu_ctx* c) {
         if (e & kErrNonFinite) return fail(c, SPHGPU_E_NONFINITE, "non-finite position, velocity or density");
         if (e & kErrOutOfRange) return fail(c, SPHGPU_E_OUT_OF_RANGE, "particle out of range");
         if (e & kErrPairCap) return fail(c, SPHGPU_E_PAIR_CAPACITY, "neighbour list capacity exceeded");
+        if (e & kErrHRange)
+            return fail(c, SPHGPU_E_OUT_OF_RANGE,
+                        "smoothing length grew past the planned search range (upload the state again)");
     }
     return 0;
 }
@@ -1965,6 +2004,10 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     if (e) return e;
     int mx = c->h_err[1];
     int cap = mx + std::max(8, mx / 2);
+    // the count was taken with the uploaded h; an evolving h may grow to the
+    // planned limit, and the neighbour count with its dim-th power
+    if (c->P.sle && c->P.hmax_upload > 0)
+        cap = (int)std::ceil(cap * std::pow(c->P.hmax / c->P.hmax_upload, c->dim));
     c->P.nbr_cap = cap;
     c->P.ldn = c->capacity;
     e = alloc_nbr(c, cap);
@@ -2051,7 +2094,11 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
             }
     for (int64_t i = 0; i < n * dim; ++i)
         if (!std::isfinite(x[i])) { why = "non-finite position"; return SPHGPU_E_NONFINITE; }
-    plan_grid_params(P, dim, hr.fast && use_verlet, n, x, h, tune);
+    double h_next_max = 0.0;
+    if (P.sle == 1)   // hsml.f90:22-25, with the uploaded density
+        for (int64_t i = 0; i < n_real; ++i)
+            if (rho[i] > 0) h_next_max = std::max(h_next_max, 2.0 * std::pow(mass[i] / rho[i], 1.0 / dim));
+    plan_grid_params(P, dim, hr.fast && use_verlet, n, x, h, tune, h_next_max);
     std::vector<std::pair<uint64_t, int64_t>> keyed(n);
     for (int64_t i = 0; i < n; ++i) {
         int64_t cc[3] = {0, 0, 0};
@@ -2471,4 +2518,5 @@ extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     delete c;
 }
 
+#include "checkpoint.hpp"
 #include "batch.hpp"

@@ -13,13 +13,13 @@
 !!   3. sphgpu_derivatives runs BGGS, density, EOS, internal/external and
 !!      artificial forces and the fixed boundary (singleStep.f90:83-142);
 !!   4. Delta, dSdt and the fields the reference updates in place (Pressure,
-!!      Density under summation density, Stress, Viscosity, the divergences)
-!!      are written back.
+!!      Density under summation density, Stress, Viscosity, the divergences,
+!!      SmoothingLength when it evolves) are written back.
 !! Neighbour lists stay on the device, so Particles%neighborNum/List/w/dwdx
 !! remain zero, as the reference leaves them on return (:199-204).
 !!
 !! Switches the GPU path does not implement (open boundary, artificial heat,
-!! shifting, averaged velocity, evolving h, density schemes other than SPH=2)
+!! shifting, averaged velocity, density schemes other than SPH=2)
 !! stop with a message instead of running something else.
 module single_step_gpu_m
     use, intrinsic :: iso_c_binding
@@ -85,9 +85,8 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
 
     if (.not. c_associated(gpu_ctx)) then
         if (Config%pa_sph /= 2) error stop "single_step (GPU): only SPH = 2 is supported"
-        if (Config%arti_heat_w .or. Config%shifting_w .or. Config%aver_velocity_w .or. &
-            Config%sle /= 0) &
-            error stop "single_step (GPU): artificial heat, shifting, averaged velocity and evolving h are not supported"
+        if (Config%arti_heat_w .or. Config%shifting_w .or. Config%aver_velocity_w) &
+            error stop "single_step (GPU): artificial heat, shifting and averaged velocity are not supported"
         cfg = sphgpu_base_config(dim)
         !! externalForce.f90:84-87 takes the wall range from particle 2 on first entry
         if (.not. sphgpu_preset(Project%nick, dim, abs(Particles(2)%x(1) - Particles(2)%x(2)), cfg)) &
@@ -103,6 +102,7 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
         cfg%artificial_stress = merge(1, 0, Config%arti_stress_w)
         cfg%kernel_correction = merge(1, 0, Config%kernel_correciton_w)
         cfg%dt = Config%delta_t
+        cfg%sle = Config%sle
         cfg%precision = SPHGPU_FP64
         call single_step_gpu_open(cfg, int(size(Particles), c_int64_t), dim)
     end if
@@ -197,6 +197,13 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
     do i = 1, ntotal
         Particles(i)%divergenceVelocity = sout(i)
     end do
+    !! h_upgrade (singleStep.f90:136, hsml.f90:8-38) ran on the GPU
+    if (Config%sle /= 0) then
+        call sphgpu_check(sphgpu_download(gpu_ctx, SPHGPU_F_H, c_loc(sout)), gpu_ctx, "download h")
+        do i = 1, ntotal
+            Particles(i)%SmoothingLength = sout(i)
+        end do
+    end if
 
     if (mod(Config%i_time_step, Config%print_interval) == 0 .and. Config%print_statistics_w) then
         call sphgpu_check(sphgpu_neighbour_stats(gpu_ctx, st), gpu_ctx, "sphgpu_neighbour_stats")

@@ -69,12 +69,14 @@ module sphgpu_m
         real(c_double) :: dt
         type(sphgpu_material) :: mat(SPHGPU_MAX_MATERIALS)
         integer(c_int32_t) :: fluid_visc_skip_solid
-        integer(c_int32_t) :: reserved
+        integer(c_int32_t) :: sle
     end type sphgpu_config
 
     public :: sphgpu_abi_version, sphgpu_create, sphgpu_upload, sphgpu_derivatives, &
               sphgpu_advance, sphgpu_download, sphgpu_export_pairs, sphgpu_neighbour_stats, &
-              sphgpu_grid_info, sphgpu_timing, sphgpu_step, sphgpu_last_error, sphgpu_destroy
+              sphgpu_grid_info, sphgpu_timing, sphgpu_step, sphgpu_last_error, sphgpu_destroy, &
+              sphgpu_checkpoint_size, sphgpu_checkpoint_save, sphgpu_checkpoint_restore, &
+              sphgpu_context_info
     public :: sphgpu_check, sphgpu_error_message, sphgpu_base_config, sphgpu_add_material, &
               sphgpu_preset, sphgpu_tait_b, sphgpu_sound_speed, f32
 
@@ -162,6 +164,34 @@ module sphgpu_m
             type(c_ptr), value :: ctx
         end subroutine sphgpu_destroy
 
+        !! Exact checkpoint: every device array and the step state
+        integer(c_int) function sphgpu_checkpoint_size(ctx, nbytes) bind(C, name="sphgpu_checkpoint_size")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(out) :: nbytes
+        end function sphgpu_checkpoint_size
+
+        integer(c_int) function sphgpu_checkpoint_save(ctx, buf, nbytes) bind(C, name="sphgpu_checkpoint_save")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx, buf
+            integer(c_int64_t), value :: nbytes
+        end function sphgpu_checkpoint_save
+
+        integer(c_int) function sphgpu_checkpoint_restore(buf, nbytes, ctx) &
+                bind(C, name="sphgpu_checkpoint_restore")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: buf
+            integer(c_int64_t), value :: nbytes
+            type(c_ptr), intent(out) :: ctx
+        end function sphgpu_checkpoint_restore
+
+        integer(c_int) function sphgpu_context_info(ctx, cfg, counts) bind(C, name="sphgpu_context_info")
+            import :: c_int, c_ptr, c_int64_t, sphgpu_config
+            type(c_ptr), value :: ctx
+            type(sphgpu_config), intent(out) :: cfg
+            integer(c_int64_t), intent(out) :: counts(3)
+        end function sphgpu_context_info
+
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -237,7 +267,7 @@ contains
         c%as_lambda = f32(0.2_8); c%as_delta = 0; c%as_exponent = 4
         c%dt = 5e-4_8
         c%fluid_visc_skip_solid = 0
-        c%reserved = 0
+        c%sle = 0
         do k = 1, SPHGPU_MAX_MATERIALS
             c%mat(k) = sphgpu_material(0, 0, 0.0_8, 0.0_8, 1.0_8, 0.0_8, 0.0_8)
         end do

@@ -38,7 +38,8 @@ EXPORTED = (
     "sphgpu_abi_version", "sphgpu_create", "sphgpu_upload", "sphgpu_derivatives",
     "sphgpu_advance", "sphgpu_download", "sphgpu_export_pairs", "sphgpu_grid_info",
     "sphgpu_timing", "sphgpu_step", "sphgpu_last_error", "sphgpu_destroy",
-    "sphgpu_neighbour_stats",
+    "sphgpu_neighbour_stats", "sphgpu_checkpoint_size", "sphgpu_checkpoint_save",
+    "sphgpu_checkpoint_restore", "sphgpu_context_info",
     "sphgpu_batch_bytes", "sphgpu_batch_create", "sphgpu_batch_add", "sphgpu_batch_advance",
     "sphgpu_batch_status", "sphgpu_batch_run_error", "sphgpu_batch_download", "sphgpu_batch_remove",
     "sphgpu_batch_info", "sphgpu_batch_timing", "sphgpu_batch_last_error", "sphgpu_batch_destroy",
@@ -83,6 +84,10 @@ def load_library(path: str = LIB_PATH):
     L.sphgpu_last_error.restype = ctypes.c_char_p
     L.sphgpu_destroy.argtypes = [vp]
     L.sphgpu_destroy.restype = None
+    L.sphgpu_checkpoint_size.argtypes = [vp, ctypes.POINTER(i64)]
+    L.sphgpu_checkpoint_save.argtypes = [vp, vp, i64]
+    L.sphgpu_checkpoint_restore.argtypes = [vp, i64, ctypes.POINTER(vp)]
+    L.sphgpu_context_info.argtypes = [vp, ctypes.POINTER(C.Config), ctypes.POINTER(i64)]
     p32 = ctypes.POINTER(ctypes.c_int32)
     p64 = ctypes.POINTER(i64)
     L.sphgpu_batch_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32, i64, i64, ctypes.c_int32,
@@ -219,6 +224,36 @@ class Simulation:
         self._check(self._L.sphgpu_timing(self._ctx, ctypes.byref(a), ctypes.byref(f),
                                           ctypes.byref(n)))
         return a.value, f.value, n.value
+
+    def checkpoint(self) -> np.ndarray:
+        """Exact checkpoint of the run (bytes as a uint8 array)."""
+        nb = ctypes.c_int64(0)
+        self._check(self._L.sphgpu_checkpoint_size(self._ctx, ctypes.byref(nb)))
+        buf = np.empty(nb.value, dtype=np.uint8)
+        self._check(self._L.sphgpu_checkpoint_save(self._ctx, buf.ctypes.data, nb.value))
+        return buf
+
+    @classmethod
+    def restore(cls, blob) -> "Simulation":
+        """A run restored from ``checkpoint()``: continues bit for bit."""
+        L = load_library()
+        blob = np.ascontiguousarray(blob, dtype=np.uint8)
+        self = cls.__new__(cls)
+        self._L = L
+        self._ctx = ctypes.c_void_p()
+        rc = L.sphgpu_checkpoint_restore(blob.ctypes.data, blob.size, ctypes.byref(self._ctx))
+        if rc != 0:
+            msg = L.sphgpu_last_error(None)
+            raise SphGpuError(rc, msg.decode() if msg else "")
+        self.cfg = C.Config()
+        counts = np.zeros(3, dtype=np.int64)
+        self._check(L.sphgpu_context_info(self._ctx, ctypes.byref(self.cfg),
+                                          counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        self.dim = int(self.cfg.dim)
+        self.n_real, self.n_dummy = int(counts[0]), int(counts[1])
+        self.n = self.n_real + self.n_dummy
+        self._keep = {}
+        return self
 
     def close(self):
         if getattr(self, "_ctx", None) and self._ctx.value:

@@ -209,6 +209,10 @@ contains
         if (stat == 0 .and. trim(buf) == "1") Config%norm_dens_w = .true.
         call get_environment_variable("REF_DT", buf, status=stat)
         if (stat == 0) read(buf, *) Config%delta_t
+        call get_environment_variable("REF_SLE", buf, status=stat)
+        if (stat == 0) read(buf, *) Config%sle
+        call get_environment_variable("REF_PAIRNUM", buf, status=stat)
+        if (stat == 0) read(buf, *) Field%pairNum
 #ifdef _OPENMP
         call get_environment_variable("REF_THREADS", buf, status=stat)
         if (stat == 0) read(buf, *) Config%nthreads

@@ -18,6 +18,7 @@
  *   external_force     src/Physic/externalForce.f90:7-174
  *   arti_visc          src/Numeric/artificialViscosity.f90:7-114
  *   artificial_stress  src/Numeric/artificialTerm.f90:7-98
+ *   h_upgrade          src/Numeric/hsml.f90:8-38
  *   single_step        src/TimeIntegration/singleStep.f90:47-144
  *   leap_frog          src/TimeIntegration/timeIntegration.f90:137-252
  *
@@ -29,6 +30,10 @@
  *   internal_force (solid), kgc, artificial_stress: compiled reference on the
  *   cantilever beams (can_beam, can_beam_3d).
  *   sum_density (+Shepard): compiled reference dam break with summation.
+ *   h_upgrade (sle 1, 2): compiled reference dam break with sle switched on.
+ *   The fluid_visc_skip_solid switch of internal_force has no reference
+ *   counterpart (it drops a term the reference keeps); with it off, the
+ *   routine is the reference's.
  *   external_force penalty and fluid-solid coupling branches: closed-form
  *   single-pair checks only (no reference case defines them; the gate case
  *   leaves its EOS sound speed uninitialised, eos.f90:35-40).
@@ -799,6 +804,21 @@ static int or_artificial_stress(const sphgpu_config* cfg, or_state* s, const or_
     return 0;
 }
 
+/* hsml.f90:8-38 (h_upgrade), real particles. */
+static void or_h_upgrade(const sphgpu_config* cfg, or_state* s) {
+    int dim = s->dim, nt = s->ntotal;
+    if (cfg->sle == 1) {
+        double factor = 2.0;
+        for (int i = 0; i < nt; i++) s->h[i] = factor * pow(s->mass[i] / s->rho[i], 1.0 / dim);
+    } else if (cfg->sle == 2) {
+        for (int i = 0; i < nt; i++) {
+            double dh = (s->h[i] / dim) * s->divv[i];
+            s->h[i] = s->h[i] + cfg->dt * dh;
+            if (s->h[i] <= 0) s->h[i] = s->h[i] - cfg->dt * dh;
+        }
+    }
+}
+
 /* singleStep.f90:47-144. Fills s->dvdt, s->drhodt (continuity only), s->dedt,
  * s->dsdt and updates rho (summation), p, eta, stress, divx, divv. */
 static int or_single_step(const sphgpu_config* cfg, or_state* s, or_pairs* keep) {
@@ -820,6 +840,7 @@ static int or_single_step(const sphgpu_config* cfg, or_state* s, or_pairs* keep)
     if (cfg->external_force) or_external_force(cfg, s, &pr, exdvdt);
     if (cfg->artificial_viscosity) or_arti_visc(cfg, s, &pr, avdvdt, avdedt);
     if (cfg->artificial_stress) rc = or_artificial_stress(cfg, s, &pr, asdvdt);
+    or_h_upgrade(cfg, s);   /* singleStep.f90:136 */
     for (int i = 0; i < nt; i++) {
         for (int d = 0; d < dim; d++) {
             int64_t q = (int64_t)i * dim + d;

@@ -38,6 +38,7 @@ from mysph_amd import cases  # noqa: E402
 DRIVER = oracle.REF_DRIVER
 SAMPLE = 16          # every 16th real particle for the dam break
 STEPS = (1, 10, 100, 1000)
+STATE_H = ("x", "v", "rho", "p", "e", "h", "divv")
 
 
 def sha(a) -> str:
@@ -156,6 +157,22 @@ def dam_break():
         arrays.update(a)
         meta.update(m)
     save("ref_dam_break_sumdens", arrays, meta)
+    dam_break_sle()
+
+
+def dam_break_sle():
+    """Smoothing-length evolution (hsml.f90:8-38) switched on in the
+    reference dam break."""
+    nt = cases.reference_dam_break()["n_real"]
+    for sle in (1, 2):
+        arrays, meta = {}, {"sample": SAMPLE, "sle": sle}
+        for n in (1, 10, 100):
+            # h = 2 dx after the first update: room for ~50 neighbours
+            d = run("dam_break", "run", n, {"REF_SLE": str(sle), "REF_PAIRNUM": "200"})
+            a, m = flat(f"step{n}_", state_record(d, nt, SAMPLE, fields=STATE_H))
+            arrays.update(a)
+            meta.update(m)
+        save(f"ref_dam_break_sle{sle}", arrays, meta)
 
 
 def beams():

@@ -8,6 +8,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import oracle
 from mysph_amd import cases
@@ -64,3 +65,13 @@ def test_dt_from_courant_form():
     c0 = 10 * np.sqrt(2 * 9.81 * 1.2)
     assert abs(c["config"].dt - 0.2 * 1.2 * 0.0075 / (c0 * 1.12)) < 1e-18
     assert abs(c["config"].dt - 3.312e-5) < 1e-8
+
+
+def test_checkpoint_restore_refuses_foreign_blobs():
+    """Header checks run before any device call: garbage and truncated
+    blobs are refused with a message."""
+    from mysph_amd.sphgpu import Simulation, SphGpuError
+    with pytest.raises(SphGpuError, match="not a checkpoint"):
+        Simulation.restore(np.zeros(4096, dtype=np.uint8))
+    with pytest.raises(SphGpuError, match="short checkpoint"):
+        Simulation.restore(np.zeros(8, dtype=np.uint8))

@@ -32,9 +32,9 @@ pytestmark = [
 STATE = ("x", "v", "rho", "p", "e")
 
 
-def run(exe, case, mode, nsteps, tmp_path, threads=8):
+def run(exe, case, mode, nsteps, tmp_path, threads=8, extra_env=None):
     out = tmp_path / f"{os.path.basename(exe)}_{case}_{mode}_{nsteps}.bin"
-    env = dict(os.environ, REF_THREADS=str(threads))
+    env = dict(os.environ, REF_THREADS=str(threads), **(extra_env or {}))
     print(f"run {os.path.basename(exe)} {case} {mode} {nsteps}", flush=True)
     r = subprocess.run([exe, case, mode, str(nsteps), str(out)], env=env, cwd=str(tmp_path),
                        capture_output=True, text=True, timeout=300)
@@ -74,3 +74,15 @@ def test_dropin_leap_frog(case, nsteps, tol, tmp_path):
     for k in STATE:
         comps = dim if k in ("x", "v") else 1
         assert_close(f"{case} {nsteps} {k}", nreal(gpu, k, comps), nreal(ref, k, comps), tol)
+
+
+@pytest.mark.parametrize("sle", [1, 2])
+def test_dropin_smoothing_length_evolution(sle, tmp_path):
+    """Config%sle on: h_upgrade (hsml.f90:8-38) runs inside the GPU
+    single_step and the updated SmoothingLength comes back."""
+    env = {"REF_SLE": str(sle), "REF_PAIRNUM": "200"}
+    ref = run(REF, "dam_break", "run", 10, tmp_path, extra_env=env)
+    gpu = run(GPU, "dam_break", "run", 10, tmp_path, extra_env=env)
+    for k in STATE + ("h",):
+        comps = 2 if k in ("x", "v") else 1
+        assert_close(f"sle {sle} {k}", nreal(gpu, k, comps), nreal(ref, k, comps), 1e-10)

@@ -163,6 +163,23 @@ def test_reference_beams(which, asw):
             assert_close(k, sim.get(k)[:nt], orc[k][:nt], tol[k])
 
 
+@pytest.mark.parametrize("sle", [1, 2])
+def test_smoothing_length_evolution(sle):
+    """h_upgrade (hsml.f90:8-38) on the GPU against the oracle (itself
+    bit-exact to the reference here): the reference dam break with sle on."""
+    c = cases.reference_dam_break()
+    cfg = c["config"].copy()
+    cfg.sle = sle
+    nt = c["n_real"]
+    orc = oracle.OracleSim(c, cfg)
+    with Simulation(c, cfg) as sim:
+        for n, tol in ((1, 1e-10), (9, 1e-10), (90, 1e-8)):
+            orc.run(n)
+            sim.advance(n)
+            for k in STATE_FIELDS + ("h",):
+                assert_close(f"sle {sle} step {sim.step - 1} {k}", sim.get(k)[:nt], orc[k][:nt], tol)
+
+
 # ------------------------------------------------- benchmark dam breaks
 
 def test_dam_break_2d_coarse_100_steps():

@@ -92,6 +92,23 @@ def test_leap_frog_steps_bit_exact(dam, nsteps):
         check_field(meta, z, f"step{nsteps}_@{k}", a, meta["sample"])
 
 
+@pytest.mark.parametrize("sle", [1, 2])
+def test_smoothing_length_evolution_bit_exact(sle):
+    """h_upgrade (hsml.f90:8-38) after the force pass, in the reference dam
+    break with sle switched on."""
+    z, meta = load(f"ref_dam_break_sle{sle}")
+    c = cases.reference_dam_break()
+    cfg = c["config"].copy()
+    cfg.sle = sle
+    nt = c["n_real"]
+    oracle.set_threads(os.cpu_count() or 1)
+    for n in (1, 10, 100):
+        s = oracle.OracleSim(c, cfg)
+        s.run(n)
+        for k in ("x", "v", "rho", "p", "e", "h", "divv"):
+            check_field(meta, z, f"step{n}_@{k}", s[k][:nt].reshape(nt, -1), meta["sample"])
+
+
 def test_summation_density_with_shepard_bit_exact():
     z, meta = load("ref_dam_break_sumdens")
     c = cases.reference_dam_break()
