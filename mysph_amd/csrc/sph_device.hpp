@@ -38,6 +38,7 @@ struct Params {
     int pair32;         // general path in mixed precision: FP64 state, FP32 pair arithmetic
     int visc_skip_solid;  // a fluid's viscous pair term skips solid neighbours
     int sle;              // smoothing-length evolution (0 none, 1 from density, 2 from div v)
+    int pa_sph;           // pair form of internal_force: 1, 2 or 3 (Riemann)
     int pair_ext;       // external pair forces can act (wall range > 0 or several fluid types)
     double g, av_alpha, av_beta, av_psi;
     double lj_factor, lj_r0, lj_p1, lj_p2;

@@ -72,6 +72,7 @@ struct Dev {
     double* asR;      // artificial-stress tensor R (2D solids, 4)
     double* kgcA;     // kernel-gradient-correction matrix A (dim*dim)
     double* rho_new;  // summation density result
+    double* pold;     // pressure before this step's EOS (SPH = 3 density rate)
     uint32_t* key;
     uint32_t* refcell;
     int* perm;
@@ -635,7 +636,14 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
         eos_mixed(P, D, s, mt.x);
     } else {
         if (real && !(mode & 2)) predict_one<DIM>(P, D, s, mt, mode & 1);
-        if (!P.summation_density) eos_th(P, D, s, mt);
+        // the Riemann density rate reads the pressures of the last EOS
+        // (con_density_riemann runs before internal_force, singleStep.f90:90-120);
+        // a count-only sizing pass leaves the state alone, so that the
+        // uploaded pressures are the ones the first pass saves
+        if (!(mode & 4)) {
+            if (P.pa_sph == 3) D.pold[s] = D.p[s];
+            if (!P.summation_density) eos_th(P, D, s, mt);
+        }
     }
     if (!real || mt.y == -1) {
         if (real) D.ncnt[s] = 0;
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
             for (int e = 0; e < DIM; ++e) {
                 double E = 2.0 * (double)T[d * DIM + e] - (d == e ? (2.0 / 3.0) * tr : 0.0);
                 sq += E * E;
-                D.Ep[o + d * DIM + e] = eta * E / (rho * rho);
+                D.Ep[o + d * DIM + e] = P.pa_sph == 2 ? eta * E / (rho * rho) : eta * E;
             }
         D.tds[s] = 0.5 * eta / rho * sq;
     } else {
@@ -885,7 +893,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
                 double E = 0.5 * ((double)T[d * DIM + e] + (double)T[e * DIM + d]);
                 D.Ed[o + d * DIM + e] = E;                                                 // edot
                 D.rot[o + d * DIM + e] = 0.5 * ((double)T[d * DIM + e] - (double)T[e * DIM + d]);  // rdot
-                D.Ep[o + d * DIM + e] = eta * E / (rho * rho);
+                D.Ep[o + d * DIM + e] = P.pa_sph == 2 ? eta * E / (rho * rho) : eta * E;
             }
     }
 }
@@ -1002,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         kernel_wg<DIM, R>(P.skf, (R)P.as_delta, (R)qi.w, f, &wdelta, &G);
     }
     R acc[3] = {0.0, 0.0, 0.0};
-    R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0;
+    R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0, drho_r = 0.0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     for (int k = 0; k < n; ++k) {
         int j = D.nbr[(int64_t)k * P.ldn + s];
@@ -1018,6 +1026,71 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         // continuity (density.f90:98-101) and divergences (divergence.f90:17-27)
         sdiv_v += (R)thj.w * dvg;
         sdiv_x += (R)thj.w * dxg;
+        // internal force, SPH = 1 (internalForce.f90:197-225) and SPH = 3,
+        // PVRS Riemann (:277-325): fluids only, solids take none
+        if (P.pa_sph != 2) {
+            if (fluid) {
+                const R rhoij = (R)1.0 / (R)(thi.x * thj.x);
+                const R pi = (R)D.p[s], pj = (R)D.p[j];
+                R a13, w[3];
+                R ex[3] = {0, 0, 0}, v_l = 0, v_r = 0;
+                const R vix[3] = {(R)vi.x, (R)vi.y, (R)vi.z}, vjx[3] = {(R)vj.x, (R)vj.y, (R)vj.z};
+                if (P.pa_sph == 3) {
+                    for (int d = 0; d < DIM; ++d) {
+                        ex[d] = -g.dx[d] / g.r;   // (x_j - x_i) / |x_j - x_i|
+                        v_l += vix[d] * ex[d];
+                        v_r += vjx[d] * ex[d];
+                    }
+                }
+                if (P.pa_sph == 1) {
+                    a13 = (pi + pj) * rhoij;
+                    for (int d = 0; d < DIM; ++d) w[d] = dv[d];
+                } else {
+                    const R Zl = (R)(thi.x * thi.z), Zr = (R)(thj.x * thj.z);
+                    const R v_ij = (Zl * v_l + Zr * v_r + (pi - pj)) / (Zl + Zr);
+                    const R p_star = (Zl * pj + Zr * pi + Zl * Zr * (v_l - v_r)) / (Zl + Zr);
+                    a13 = (R)2 * p_star * rhoij;
+                    for (int d = 0; d < DIM; ++d) {
+                        const R vs = v_ij * ex[d] + ((vix[d] + vjx[d]) / (R)2 - ((v_l + v_r) / (R)2) * ex[d]);
+                        w[d] = (R)2 * (vix[d] - vs);
+                    }
+                }
+                const int tvj = D.meta[j].x;
+                const bool vj_on = visc && !(P.visc_skip_solid && (tvj > 100 || tvj < -100));
+                R dot = (R)0;
+                for (int d = 0; d < DIM; ++d) {
+                    R mv = (R)0.0;   // matmul(eta_i edot_i + eta_j edot_j, dwdx)
+                    if (vj_on) {
+                        const double* Ej = D.Ep + (int64_t)j * DIM * DIM;
+                        for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + (R)Ej[d * DIM + e]) * g.gw[e];
+                    }
+                    acc[d] += mj * (-a13 * g.gw[d] + mv * rhoij);
+                    dot += w[d] * g.gw[d];
+                }
+                de_in += mj * a13 * dot;
+            }
+            if (P.pa_sph == 3) {
+                // density.f90:109-145 with the pressures of the last EOS and
+                // the neighbour's sound speed in both impedances
+                const R cj = (R)thj.z;
+                const R Zl = (R)thi.x * cj, Zr = (R)thj.x * cj;
+                const R pi = (R)D.pold[s], pj = (R)D.pold[j];
+                R v_l = 0, v_r = 0, ex[3] = {0, 0, 0};
+                const R vix[3] = {(R)vi.x, (R)vi.y, (R)vi.z}, vjx[3] = {(R)vj.x, (R)vj.y, (R)vj.z};
+                for (int d = 0; d < DIM; ++d) {
+                    ex[d] = -g.dx[d] / g.r;
+                    v_l += vix[d] * ex[d];
+                    v_r += vjx[d] * ex[d];
+                }
+                const R v_ij = (Zl * v_l + Zr * v_r + (pi - pj)) / (Zl + Zr);
+                R dot = (R)0;
+                for (int d = 0; d < DIM; ++d) {
+                    const R vs = v_ij * ex[d] + ((vix[d] + vjx[d]) / (R)2 - ((v_l + v_r) / (R)2) * ex[d]);
+                    dot += (vix[d] - vs) * g.gw[d];
+                }
+                drho_r += (R)2 * (R)thi.x * mj / (R)thj.x * dot;
+            }
+        } else {
         // internal force, SPH = 2 (internalForce.f90:228-274)
         R aux = (R)(thi.y + thj.y);
         if (fluid) {
@@ -1040,6 +1113,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
             }
         }
         de_in += mj * aux * dvg;
+        }
         // external pair forces (externalForce.f90:103-174)
         if (P.external_force) {
             const int tj = D.meta[j].x;
@@ -1102,6 +1176,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         ac[DIM - 1] += P.g;
     if (mt.z == 1) ac[0] = ac[1] = ac[2] = 0.0;   // fixedBoundary, boundaryCondition.f90:32-45
     double drho = P.summation_density ? 0.0 : thi.x * (double)sdiv_v;
+    if (P.pa_sph == 3) drho = (double)drho_r;
     D.dv[s] = make_double4(ac[0], ac[1], ac[2], drho);
     double tds = (P.viscosity || !fluid) ? D.tds[s] : 0.0;
     D.de[s] = tds + 0.5 * (double)de_in + (double)de_av;
@@ -1452,7 +1527,18 @@ static void free_all(sphgpu_ctx* c) {
 static int validate_config(const sphgpu_config* cfg, std::string* why) {
     if (cfg->dim != 2 && cfg->dim != 3) { *why = "dim must be 2 or 3"; return SPHGPU_E_BAD_CONFIG; }
     if (cfg->skf < 1 || cfg->skf > 3) { *why = "skf must be 1, 2 or 3"; return SPHGPU_E_BAD_CONFIG; }
-    if (cfg->pa_sph != 2) { *why = "only SPH = 2 (internalForce.f90:228-274) is supported"; return SPHGPU_E_BAD_CONFIG; }
+    if (cfg->pa_sph == 4) {
+        // sum_density_dsph (density.f90:147-221) carries `ratio` and `s` from
+        // one particle to the next and reads densities it has already
+        // overwritten: its result depends on the particle order by design
+        *why = "SPH = 4 (DSPH) is an order-dependent serial scheme; SPH = 1, 2, 3 are supported";
+        return SPHGPU_E_BAD_CONFIG;
+    }
+    if (cfg->pa_sph < 1 || cfg->pa_sph > 3) { *why = "SPH must be 1, 2 or 3"; return SPHGPU_E_BAD_CONFIG; }
+    if (cfg->pa_sph == 3 && cfg->summation_density) {
+        *why = "SPH = 3 with summation density: the reference computes the Riemann rate and never applies it";
+        return SPHGPU_E_BAD_CONFIG;
+    }
     if (cfg->artificial_stress && cfg->dim != 2) {
         *why = "artificial stress is 2D only (artificialTerm.f90:94-95)";
         return SPHGPU_E_BAD_CONFIG;
@@ -1557,6 +1643,7 @@ static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_
     P.pair32 = 0;
     P.visc_skip_solid = g.fluid_visc_skip_solid != 0;
     P.sle = g.sle;
+    P.pa_sph = g.pa_sph;
 }
 
 // The fused fluid force kernel covers the dam-break physics set; anything
@@ -1564,7 +1651,8 @@ static void fill_params_cfg(Params& P, const sphgpu_config& g, int64_t n, int64_
 // artificial stress, Gauss/quintic kernels) takes the general kernels.
 static bool fast_path_ok(const sphgpu_config& g, const Params& P, bool force_general) {
     return g.skf == 1 && !g.viscosity && !P.has_solid && !g.kernel_correction &&
-           !g.summation_density && !g.artificial_stress && g.sle == 0 && !force_general;
+           !g.summation_density && !g.artificial_stress && g.sle == 0 && g.pa_sph == 2 &&
+           !force_general;
 }
 
 static bool fluid_fast_path(const sphgpu_ctx* c) { return fast_path_ok(c->cfg, c->P, c->force_general); }
@@ -1940,6 +2028,7 @@ static int alloc_state(sphgpu_ctx* c) {
         rc |= dalloc(c, &D.drho_prev, N); rc |= dalloc(c, &D.dvf, N); rc |= dalloc(c, &D.def, N);
     }
     if (c->cfg.summation_density) rc |= dalloc(c, &D.rho_new, N);
+    if (c->cfg.pa_sph == 3) rc |= dalloc(c, &D.pold, N);
     int nb = (int)((N + kBlock - 1) / kBlock);
     rc |= dalloc(c, &D.partial, (size_t)nb * 6);
     rc |= dalloc(c, &D.grid, 1);

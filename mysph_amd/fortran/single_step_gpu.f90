@@ -19,7 +19,7 @@
 !! remain zero, as the reference leaves them on return (:199-204).
 !!
 !! Switches the GPU path does not implement (open boundary, artificial heat,
-!! shifting, averaged velocity, density schemes other than SPH=2)
+!! shifting, averaged velocity, the DSPH density scheme SPH=4)
 !! stop with a message instead of running something else.
 module single_step_gpu_m
     use, intrinsic :: iso_c_binding
@@ -84,7 +84,7 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
     N = ntotal + ndummy
 
     if (.not. c_associated(gpu_ctx)) then
-        if (Config%pa_sph /= 2) error stop "single_step (GPU): only SPH = 2 is supported"
+        if (Config%pa_sph == 4) error stop "single_step (GPU): SPH = 4 (DSPH) is not supported"
         if (Config%arti_heat_w .or. Config%shifting_w .or. Config%aver_velocity_w) &
             error stop "single_step (GPU): artificial heat, shifting and averaged velocity are not supported"
         cfg = sphgpu_base_config(dim)

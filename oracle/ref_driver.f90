@@ -66,6 +66,7 @@ program ref_driver
         error stop "unknown case"
     end select
     if (has_in) call read_velocity_override()
+    call apply_dummy_sound_speed(ntotal)
 
     ndummy = 0
     nbuffer = 0
@@ -213,11 +214,30 @@ contains
         if (stat == 0) read(buf, *) Config%sle
         call get_environment_variable("REF_PAIRNUM", buf, status=stat)
         if (stat == 0) read(buf, *) Field%pairNum
+        call get_environment_variable("REF_PA_SPH", buf, status=stat)
+        if (stat == 0) read(buf, *) Config%pa_sph
 #ifdef _OPENMP
         call get_environment_variable("REF_THREADS", buf, status=stat)
         if (stat == 0) read(buf, *) Config%nthreads
 #endif
     end subroutine apply_env_overrides
+
+    !! REF_DUMMY_C: SoundSpeed of every slot past the real particles. The dummy
+    !! generators never set it (dummyParticles.f90:544-659), so the value
+    !! survives their per-step regeneration; the Riemann schemes (SPH = 3)
+    !! divide by rho*c summed over a pair and need it nonzero.
+    subroutine apply_dummy_sound_speed(nreal)
+        integer, intent(in) :: nreal
+        character(len=64) :: buf
+        integer :: stat, i
+        real(8) :: c
+        call get_environment_variable("REF_DUMMY_C", buf, status=stat)
+        if (stat /= 0) return
+        read(buf, *) c
+        do i = nreal + 1, size(P)
+            P(i)%SoundSpeed = c
+        end do
+    end subroutine apply_dummy_sound_speed
 
     subroutine read_velocity_override()
         integer :: iu, d

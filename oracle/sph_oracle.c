@@ -526,7 +526,89 @@ static void or_divergence(or_state* s, const or_pairs* pr) {
     }
 }
 
-/* internalForce.f90:10-342 with SOLID=1 and Shear present, SPH=2. */
+/* Unit vector from i to j (norm2 of the separation) and the Riemann
+ * interface state of density.f90:120-141 / internalForce.f90:285-305.
+ * Z_l, Z_r: acoustic impedances; pi, pj: the pressures it is given. */
+static void or_riemann(const or_state* s, int i, int j, double Z_l, double Z_r, double pi, double pj,
+                       double* e, double* v_l, double* v_r, double* v_ij, double* v_star) {
+    int dim = s->dim;
+    const double* xi = s->x + (int64_t)i * dim;
+    const double* xj = s->x + (int64_t)j * dim;
+    const double* vi = s->v + (int64_t)i * dim;
+    const double* vj = s->v + (int64_t)j * dim;
+    double ss = 0;
+    for (int d = 0; d < dim; d++) ss = ss + (xj[d] - xi[d]) * (xj[d] - xi[d]);
+    double nrm = sqrt(ss);
+    for (int d = 0; d < dim; d++) e[d] = (xj[d] - xi[d]) / nrm;
+    double a = 0, b = 0;
+    for (int d = 0; d < dim; d++) { a = a + vi[d] * e[d]; b = b + vj[d] * e[d]; }
+    *v_l = a;
+    *v_r = b;
+    *v_ij = (Z_l * a + Z_r * b + (pi - pj)) / (Z_l + Z_r);
+    for (int d = 0; d < dim; d++)
+        v_star[d] = *v_ij * e[d] + ((vi[d] + vj[d]) / 2 - ((a + b) / 2) * e[d]);
+}
+
+/* Fluid pair sums of SPH = 1 (internalForce.f90:197-225) and SPH = 3, the
+ * PVRS Riemann form (:277-325), for particle i. */
+static void or_pair_sph13(const sphgpu_config* cfg, const or_state* s, const or_pairs* pr,
+                          const double* edot, int i, double* acc, double* de) {
+    int dim = s->dim;
+    const double* vi = s->v + (int64_t)i * dim;
+    const double* Ei = edot + (int64_t)i * dim * dim;
+    for (int64_t k = pr->off[i]; k < pr->off[i + 1]; k++) {
+        int j = pr->nbr[k];
+        const double* dw = pr->dwdx + k * dim;
+        const double* vj = s->v + (int64_t)j * dim;
+        const double* Ej = edot + (int64_t)j * dim * dim;
+        double rhoij = 1.0 / (s->rho[i] * s->rho[j]);
+        double aux, dv[3];
+        if (cfg->pa_sph == 1) {
+            aux = (s->p[i] + s->p[j]) * rhoij;
+            for (int d = 0; d < dim; d++) dv[d] = vi[d] - vj[d];
+        } else {
+            double Z_l = s->rho[i] * s->c[i], Z_r = s->rho[j] * s->c[j];
+            double e[3], v_l, v_r, v_ij, v_star[3];
+            or_riemann(s, i, j, Z_l, Z_r, s->p[i], s->p[j], e, &v_l, &v_r, &v_ij, v_star);
+            double p_star = (Z_l * s->p[j] + Z_r * s->p[i] + Z_l * Z_r * (v_l - v_r)) / (Z_l + Z_r);
+            aux = 2 * p_star * rhoij;
+            for (int d = 0; d < dim; d++) dv[d] = 2 * (vi[d] - v_star[d]);
+        }
+        for (int d = 0; d < dim; d++) {
+            double mv = 0;   /* matmul(eta_i edot_i + eta_j edot_j, dwdx) */
+            for (int e = 0; e < dim; e++)
+                mv = mv + (s->eta[i] * Ei[d * dim + e] + s->eta[j] * Ej[d * dim + e]) * dw[e];
+            acc[d] = acc[d] + s->mass[j] * (-aux * dw[d] + mv * rhoij);
+        }
+        double dot = 0;
+        for (int d = 0; d < dim; d++) dot = dot + dv[d] * dw[d];
+        *de = *de + s->mass[j] * aux * dot;
+    }
+}
+
+/* density.f90:109-145, PVRS Riemann solver. Quirk kept: Z_l takes the
+ * neighbour's sound speed (P(j)%SoundSpeed), and the pressures are the ones
+ * stored before this step's EOS. */
+static void or_con_density_riemann(or_state* s, const or_pairs* pr, double* drhodt) {
+    int dim = s->dim;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < s->ntotal; i++) {
+        double acc = 0;
+        const double* vi = s->v + (int64_t)i * dim;
+        for (int64_t k = pr->off[i]; k < pr->off[i + 1]; k++) {
+            int j = pr->nbr[k];
+            double Z_l = s->rho[i] * s->c[j], Z_r = s->rho[j] * s->c[j];
+            double e[3], v_l, v_r, v_ij, v_star[3];
+            or_riemann(s, i, j, Z_l, Z_r, s->p[i], s->p[j], e, &v_l, &v_r, &v_ij, v_star);
+            double dot = 0;
+            for (int d = 0; d < dim; d++) dot = dot + (vi[d] - v_star[d]) * pr->dwdx[k * dim + d];
+            acc = acc + 2 * s->rho[i] * s->mass[j] / s->rho[j] * dot;
+        }
+        drhodt[i] = acc;
+    }
+}
+
+/* internalForce.f90:10-342 with SOLID=1 and Shear present, SPH = 1, 2, 3. */
 static void or_internal_force(const sphgpu_config* cfg, or_state* s, const or_pairs* pr,
                               double* dvdt, double* dedt) {
     int dim = s->dim, N = s->n, nt = s->ntotal;
@@ -618,7 +700,10 @@ static void or_internal_force(const sphgpu_config* cfg, or_state* s, const or_pa
         double acc[3] = {0, 0, 0}, de = 0;
         const double* vi = s->v + (int64_t)i * dim;
         double ri2 = s->rho[i] * s->rho[i];
-        if (ai <= 100) {
+        if (cfg->pa_sph != 2) {
+            /* SPH = 1 (:197-225) and 3 (:277-325): fluids only */
+            if (ai <= 100) or_pair_sph13(cfg, s, pr, edot, i, acc, &de);
+        } else if (ai <= 100) {
             const double* Ei = edot + (int64_t)i * dim * dim;
             for (int64_t k = pr->off[i]; k < pr->off[i + 1]; k++) {
                 int j = pr->nbr[k];
@@ -833,7 +918,10 @@ static int or_single_step(const sphgpu_config* cfg, or_state* s, or_pairs* keep)
     double* indedt = (double*)calloc((size_t)nt + 1, sizeof(double));
     double* avdedt = (double*)calloc((size_t)nt + 1, sizeof(double));
     if (cfg->kernel_correction) or_kgc(s, &pr);
-    if (cfg->summation_density) or_sum_density(cfg, s, &pr);
+    /* singleStep.f90:90-105: SPH = 3 takes the Riemann rate whatever the
+     * summation switch says */
+    if (cfg->pa_sph == 3) or_con_density_riemann(s, &pr, s->drhodt);
+    else if (cfg->summation_density) or_sum_density(cfg, s, &pr);
     else or_con_density(s, &pr, s->drhodt);
     or_divergence(s, &pr);
     or_internal_force(cfg, s, &pr, indvdt, indedt);

@@ -160,6 +160,28 @@ def dam_break():
     dam_break_sle()
 
 
+def dam_break_sph13():
+    """The other pair forms of internal_force: SPH = 1 (internalForce.f90:
+    197-225) and SPH = 3 with the Riemann density rate (:277-325,
+    density.f90:109-145), in the reference dam break with its dummies given
+    the water's sound speed (50, eos.f90:37; the Riemann forms divide by
+    rho_i c + rho_j c, which is 0/0 for a dummy pair at c = 0)."""
+    nt = cases.reference_dam_break()["n_real"]
+    for sph in (1, 3):
+        arrays, meta = {}, {"sample": SAMPLE, "pa_sph": sph, "dummy_c": 50.0}
+        d = run("dam_break", "deriv", 1, {"REF_PA_SPH": str(sph), "REF_DUMMY_C": "50"})
+        for k, comp in (("dvdt", 2), ("drhodt", 1), ("dedt", 1)):
+            a = d[k].reshape(-1, comp)[:nt]
+            arrays["deriv_" + k] = a[::SAMPLE].copy()
+            meta["deriv_sha_" + k] = sha(a)
+        for n in (1, 10, 100):
+            d = run("dam_break", "run", n, {"REF_PA_SPH": str(sph), "REF_DUMMY_C": "50"})
+            a, m = flat(f"step{n}_", state_record(d, nt, SAMPLE, fields=STATE_H))
+            arrays.update(a)
+            meta.update(m)
+        save(f"ref_dam_break_sph{sph}", arrays, meta)
+
+
 def dam_break_sle():
     """Smoothing-length evolution (hsml.f90:8-38) switched on in the
     reference dam break."""

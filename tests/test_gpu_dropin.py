@@ -86,3 +86,16 @@ def test_dropin_smoothing_length_evolution(sle, tmp_path):
     for k in STATE + ("h",):
         comps = 2 if k in ("x", "v") else 1
         assert_close(f"sle {sle} {k}", nreal(gpu, k, comps), nreal(ref, k, comps), 1e-10)
+
+
+@pytest.mark.parametrize("sph", [1, 3])
+def test_dropin_sph_forms(sph, tmp_path):
+    """Config%pa_sph = 1 and 3 (Riemann) through the GPU single_step; the
+    dummies carry the water's sound speed (the Riemann forms divide by
+    rho_i c + rho_j c)."""
+    env = {"REF_PA_SPH": str(sph), "REF_DUMMY_C": "50"}
+    ref = run(REF, "dam_break", "run", 10, tmp_path, extra_env=env)
+    gpu = run(GPU, "dam_break", "run", 10, tmp_path, extra_env=env)
+    for k in STATE:
+        comps = 2 if k in ("x", "v") else 1
+        assert_close(f"sph {sph} {k}", nreal(gpu, k, comps), nreal(ref, k, comps), 1e-10)

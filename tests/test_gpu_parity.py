@@ -163,6 +163,31 @@ def test_reference_beams(which, asw):
             assert_close(k, sim.get(k)[:nt], orc[k][:nt], tol[k])
 
 
+@pytest.mark.parametrize("sph", [1, 3])
+def test_sph_forms_1_and_3(sph):
+    """internal_force SPH = 1 and SPH = 3 (PVRS Riemann, with the Riemann
+    density rate) on the GPU against the oracle, in the reference dam break
+    with its dummies at the water's sound speed."""
+    c = cases.reference_dam_break()
+    c["c"][c["n_real"]:] = 50.0
+    cfg = c["config"].copy()
+    cfg.pa_sph = sph
+    nt = c["n_real"]
+    orc = oracle.OracleSim(c, cfg)
+    with Simulation(c, cfg) as sim:
+        sim.derivatives()
+        orc.derivatives()
+        for k in ("dvdt", "drhodt", "dedt"):
+            assert_close(f"sph {sph} d1 {k}", sim.get(k)[:nt], orc[k][:nt], 1e-10)
+    orc = oracle.OracleSim(c, cfg)
+    with Simulation(c, cfg) as sim:
+        for n, tol in ((10, 1e-10), (90, 1e-8)):
+            orc.run(n)
+            sim.advance(n)
+            for k in STATE_FIELDS:
+                assert_close(f"sph {sph} step {sim.step - 1} {k}", sim.get(k)[:nt], orc[k][:nt], tol)
+
+
 @pytest.mark.parametrize("sle", [1, 2])
 def test_smoothing_length_evolution(sle):
     """h_upgrade (hsml.f90:8-38) on the GPU against the oracle (itself

@@ -92,6 +92,42 @@ def test_leap_frog_steps_bit_exact(dam, nsteps):
         check_field(meta, z, f"step{nsteps}_@{k}", a, meta["sample"])
 
 
+@pytest.mark.parametrize("sph", [1, 3])
+def test_sph_forms_1_and_3(sph):
+    """internal_force SPH = 1 and SPH = 3 with the Riemann density rate, in
+    the reference dam break (dummies at c = 50)."""
+    z, meta = load(f"ref_dam_break_sph{sph}")
+    c = cases.reference_dam_break()
+    c["c"][c["n_real"]:] = meta["dummy_c"]
+    cfg = c["config"].copy()
+    cfg.pa_sph = sph
+    nt = c["n_real"]
+    oracle.set_threads(os.cpu_count() or 1)
+    s = oracle.OracleSim(c, cfg)
+    s.derivatives()
+    for k in ("dvdt", "drhodt", "dedt"):
+        check_field(meta, z, f"deriv_@{k}", s[k][:nt].reshape(nt, -1), meta["sample"])
+    for n in (1, 10, 100):
+        s = oracle.OracleSim(c, cfg)
+        s.run(n)
+        for k in ("x", "v", "rho", "p", "e", "divv"):
+            full = s[k][:nt].reshape(nt, -1)
+            if sph == 1 or n == 1:
+                check_field(meta, z, f"step{n}_@{k}", full, meta["sample"])
+            else:
+                # Two last-bit sources once particles leave the lattice: the
+                # Riemann rate reads the dummies' pressure, which the
+                # reference regenerates every step (hydrostatic p,
+                # dummyParticles.f90:578-584) and the oracle re-evaluates
+                # from their density; and e_ij divides by Fortran norm2,
+                # whose runtime may scale the sum of squares where the
+                # oracle takes sqrt of the plain sum. Sampled values within
+                # 1e-12 after 100 steps (measured 2.6e-13 at most).
+                ref = z[f"step{n}_{k}"]
+                got = full[::meta["sample"]].reshape(ref.shape)
+                assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max(), (n, k)
+
+
 @pytest.mark.parametrize("sle", [1, 2])
 def test_smoothing_length_evolution_bit_exact(sle):
     """h_upgrade (hsml.f90:8-38) after the force pass, in the reference dam
