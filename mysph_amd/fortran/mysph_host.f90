@@ -14,7 +14,7 @@
 !! sph.toml keys read (the reference's, with its defaults):
 !!   name, nick
 !!   [Parameter] DIM, SKF, deltaT, maxTimeStep (1000), saveInterval (500),
-!!     printInterval (100), printStatistics (true), summationDensity,
+!!     printInterval (100), printStatistics (true), SPH (2), SLE (0), summationDensity,
 !!     normalizeDensity, viscosity, gravity, externalForce,
 !!     artificialViscosity, artificialStress, kernelCorrection
 !!   [postProcess] writeVTKfile (true), writeDummyParticle (true)
@@ -29,6 +29,11 @@
 !! --setup-only <file>: build the case, write it to <file> (and the step-0
 !! output files unless --no-output) and stop without touching the GPU.
 !! --final <file>: write the final state to <file>.
+!! --checkpoint <file>: write an exact checkpoint of the run at the end
+!! (sphgpu_checkpoint_save).
+!! --resume <file>: continue from such a checkpoint (the case is still built
+!! from sph.toml for its particle types and counts, which must match) up to
+!! maxTimeStep; the result equals an uninterrupted run bit for bit.
 !! --abi: print the library's ABI version and the size of sphgpu_config as
 !! this binding lays it out, then stop.
 program mysph_host
@@ -50,12 +55,14 @@ program mysph_host
 
     character(len=1024) :: arg
     character(len=:), allocatable :: project_dir, setup_only, final_path, msg, name, nick, kind, &
-                                     prec, out_dir, vtk_dir, file
-    logical :: ok, no_output, write_vtk, write_dummy, print_stats
+                                     prec, out_dir, vtk_dir, file, ckpt_out, ckpt_in
+    logical :: ok, no_output, write_vtk, write_dummy, print_stats, report
     integer :: i, nargs, dim, layers, nc, nt, max_step, save_every, print_every, step, chunk, n
     integer(int64) :: seed, t0, t1, rate, st4(4)
     real(8) :: dx, h_over_dx, jitter, alpha, column(3), tank(3), secs, adv_ms, force_ms
-    integer(c_int64_t) :: launches
+    integer(c_int64_t) :: launches, nbytes, counts(3)
+    integer(c_int8_t), allocatable, target :: blob(:)
+    type(sphgpu_config) :: cfg_ck
     type(particle_set), target :: ps
     type(sphgpu_config) :: cfg
     type(c_ptr) :: ctx
@@ -64,6 +71,8 @@ program mysph_host
     project_dir = "."
     setup_only = ""
     final_path = ""
+    ckpt_out = ""
+    ckpt_in = ""
     no_output = .false.
     nargs = command_argument_count()
     i = 1
@@ -78,6 +87,14 @@ program mysph_host
             i = i + 1
             call get_command_argument(i, arg)
             final_path = trim(arg)
+        case ("--checkpoint")
+            i = i + 1
+            call get_command_argument(i, arg)
+            ckpt_out = trim(arg)
+        case ("--resume")
+            i = i + 1
+            call get_command_argument(i, arg)
+            ckpt_in = trim(arg)
         case ("--no-output")
             no_output = .true.
         case ("--abi")
@@ -169,26 +186,41 @@ program mysph_host
         stop
     end if
 
-    call sphgpu_check(sphgpu_create(cfg, int(n, c_int64_t), ctx), ctx, "sphgpu_create")
-    call sphgpu_check(sphgpu_upload(ctx, int(ps%n_real, c_int64_t), int(ps%n_dummy, c_int64_t), &
-                      c_loc(ps%x), c_loc(ps%v), c_loc(ps%mass), c_loc(ps%rho), c_loc(ps%p), &
-                      c_loc(ps%e), c_loc(ps%c), c_loc(ps%h), c_null_ptr, c_loc(ps%type), &
-                      c_loc(ps%state), c_loc(ps%boundary)), ctx, "sphgpu_upload")
+    step = 0
+    if (len(ckpt_in) > 0) then
+        call read_blob(ckpt_in, blob)
+        call sphgpu_check(sphgpu_checkpoint_restore(c_loc(blob), int(size(blob), c_int64_t), ctx), &
+                          c_null_ptr, "sphgpu_checkpoint_restore")
+        deallocate(blob)
+        call sphgpu_check(sphgpu_context_info(ctx, cfg_ck, counts), ctx, "sphgpu_context_info")
+        if (counts(1) /= ps%n_real .or. counts(2) /= ps%n_dummy) &
+            error stop "mysph_host: the checkpoint holds another particle set than sph.toml builds"
+        step = int(sphgpu_step(ctx)) - 1
+        write(*, "(A, I0)") " Resumed from " // ckpt_in // " after step ", step
+    else
+        call sphgpu_check(sphgpu_create(cfg, int(n, c_int64_t), ctx), ctx, "sphgpu_create")
+        call sphgpu_check(sphgpu_upload(ctx, int(ps%n_real, c_int64_t), int(ps%n_dummy, c_int64_t), &
+                          c_loc(ps%x), c_loc(ps%v), c_loc(ps%mass), c_loc(ps%rho), c_loc(ps%p), &
+                          c_loc(ps%e), c_loc(ps%c), c_loc(ps%h), c_null_ptr, c_loc(ps%type), &
+                          c_loc(ps%state), c_loc(ps%boundary)), ctx, "sphgpu_upload")
+    end if
     call frame_allocate(frame, dim, n)
-    if (.not. no_output .and. save_every > 0) then
+    if (.not. no_output .and. save_every > 0 .and. step == 0) then
         call download(ctx, frame)
         call write_output(out_dir, vtk_dir, 0, frame, write_vtk, write_dummy)
     end if
 
     call system_clock(t0, rate)
-    step = 0
     do while (step < max_step)
         chunk = max_step - step
         if (print_every > 0) chunk = min(chunk, print_every - mod(step, print_every))
         if (save_every > 0) chunk = min(chunk, save_every - mod(step, save_every))
         call sphgpu_check(sphgpu_advance(ctx, int(chunk, c_int32_t)), ctx, "sphgpu_advance")
         step = step + chunk
-        if (print_every > 0 .and. mod(step, print_every) == 0) then
+        !! Fortran does not short-circuit .and.: mod by zero would trap
+        report = .false.
+        if (print_every > 0) report = mod(step, print_every) == 0
+        if (report) then
             call sphgpu_check(sphgpu_timing(ctx, adv_ms, force_ms, launches), ctx, "sphgpu_timing")
             write(*, "(A, I0, A, I0, A, F9.3, A, F8.3, A)") " step ", step, " / ", max_step, &
                 ": ", adv_ms / chunk, " ms/step (force kernel ", force_ms, " ms)"
@@ -219,9 +251,39 @@ program mysph_host
         ps%e = frame%e
         call write_dump(final_path, ps)
     end if
+    if (len(ckpt_out) > 0) then
+        call sphgpu_check(sphgpu_checkpoint_size(ctx, nbytes), ctx, "sphgpu_checkpoint_size")
+        allocate(blob(nbytes))
+        call sphgpu_check(sphgpu_checkpoint_save(ctx, c_loc(blob), nbytes), ctx, "sphgpu_checkpoint_save")
+        call write_blob(ckpt_out, blob)
+        write(*, "(A, I0, A)") " Checkpoint written to " // ckpt_out // " (", nbytes, " bytes)"
+    end if
     call sphgpu_destroy(ctx)
 
 contains
+
+    subroutine write_blob(path, b)
+        character(len=*), intent(in) :: path
+        integer(c_int8_t), intent(in) :: b(:)
+        integer :: u
+        open(newunit=u, file=path, access="stream", form="unformatted", status="replace", action="write")
+        write(u) b
+        close(u)
+    end subroutine write_blob
+
+    subroutine read_blob(path, b)
+        character(len=*), intent(in) :: path
+        integer(c_int8_t), allocatable, intent(out) :: b(:)
+        integer :: u, ios
+        integer(int64) :: sz
+        open(newunit=u, file=path, access="stream", form="unformatted", status="old", action="read", &
+             iostat=ios)
+        if (ios /= 0) error stop "mysph_host: cannot open the checkpoint file"
+        inquire(unit=u, size=sz)
+        allocate(b(sz))
+        read(u) b
+        close(u)
+    end subroutine read_blob
 
     integer function precision_code(p)
         character(len=*), intent(in) :: p
@@ -240,6 +302,7 @@ contains
         type(sphgpu_config), intent(inout) :: c
         if (toml_has("Parameter.SKF")) c%skf = toml_int("Parameter.SKF", 1)
         if (toml_has("Parameter.SPH")) c%pa_sph = toml_int("Parameter.SPH", 2)
+        if (toml_has("Parameter.SLE")) c%sle = toml_int("Parameter.SLE", 0)
         if (toml_has("Parameter.deltaT")) c%dt = toml_real("Parameter.deltaT", c%dt)
         call flag("Parameter.summationDensity", c%summation_density)
         call flag("Parameter.normalizeDensity", c%normalize_density)

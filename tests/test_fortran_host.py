@@ -154,3 +154,4 @@ def test_missing_project_file(tmp_path):
     (tmp_path / "empty").mkdir()
     r = run_host(tmp_path / "empty", check=False)
     assert r.returncode != 0 and "sph.toml" in (r.stdout + r.stderr)
+

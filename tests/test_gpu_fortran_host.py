@@ -89,3 +89,36 @@ file = "ref.bin"
     for k in FIELDS:
         comps = 2 if k in ("x", "v") else 1
         assert_close(f"host {k}", f[k][: nt * comps], orc[k][:nt].ravel(), 1e-10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_checkpoint_resume_equals_uninterrupted_run(precision, tmp_path):
+    """--checkpoint after 35 steps, --resume to 80, against one 80-step run:
+    the final states are bit-identical (sphgpu_checkpoint_save / _restore
+    through the Fortran binding)."""
+    def toml(steps):
+        write_toml(tmp_path, f"""
+name = "2D dam break"
+nick = "dam_break_2d"
+[Parameter]
+DIM = 2
+maxTimeStep = {steps}
+saveInterval = 0
+printInterval = 0
+[Case]
+kind = "dam_break"
+dx = 0.04
+precision = "{precision}"
+""")
+    print(f"mysph_host checkpoint/resume {precision}", flush=True)
+    toml(80)
+    run_host(tmp_path, "--no-output", "--final", tmp_path / "straight.bin")
+    toml(35)
+    run_host(tmp_path, "--no-output", "--checkpoint", tmp_path / "ck.bin")
+    toml(80)
+    r = run_host(tmp_path, "--no-output", "--resume", tmp_path / "ck.bin", "--final", tmp_path / "resumed.bin")
+    assert "after step 35" in r.stdout
+    a = oracle.read_dump(str(tmp_path / "straight.bin"))
+    b = oracle.read_dump(str(tmp_path / "resumed.bin"))
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
