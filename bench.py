@@ -34,6 +34,25 @@ PEAK_HBM = 8.0e12    # B/s, MI355X HBM3E
 # scaled by the same counters on a plain 1 GiB copy (counters are not readable
 # from inside a process, so the figure is recorded beside the kernel trace).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "force_kernel_hbm_traffic.json")
+# Algorithmic bytes per particle-step of the whole step (each SoA field read
+# and written once per kernel, neighbour re-reads excluded): neighbour search,
+# fused pair kernel, integrator; and FLOP per particle-step outside the pairs.
+STEP_BYTES = {"mixed": (190, 60, 250), "fp64": (217, 96, 328)}
+FLOP_PER_PARTICLE = 40
+
+
+def step_roofline(n_fluid, pairs, ms_per_step, precision):
+    """Ideal step time T* = sum over the three stages of max(bytes / HBM peak,
+    FLOP / VALU peak), against the measured step."""
+    nnps, pair_b, integ = STEP_BYTES[precision]
+    valu = (PEAK_FP32 if precision == "mixed" else PEAK_FP64) * 1e12
+    t = [nnps * n_fluid / PEAK_HBM,
+         max(pair_b * n_fluid / PEAK_HBM, FLOP_PER_PAIR_3D * pairs / valu),
+         max(integ * n_fluid / PEAK_HBM, FLOP_PER_PARTICLE * n_fluid / valu)]
+    t_star = 1e3 * sum(t)
+    return {"t_star_ms": t_star, "ms_per_step": ms_per_step, "frac": t_star / ms_per_step,
+            "stages_ms": {"nnps": 1e3 * t[0], "pair": 1e3 * t[1], "integrator": 1e3 * t[2]},
+            "bytes_per_particle_step": nnps + pair_b + integ}
 
 
 def measured_traffic(mixed):
@@ -287,6 +306,8 @@ def main():
                                   if traffic and head["force_ms"] > 0 else None),
         },
     }
+    prec_head = "mixed" if "mixed" in res else "fp64"
+    out["step_roofline"] = step_roofline(nf, head["pairs"], out["ms_per_step"], prec_head)
     if "fp64" in res:
         out["fp64"] = {"value": res["fp64"]["pss"],
                        "ms_per_step": 1e3 * res["fp64"]["wall_s"] / args.steps,
