@@ -1,0 +1,127 @@
+"""Edge cases of the hot path against the oracle, on both pipelines (the
+fused fast path and the general path) and in both precisions where they
+apply:
+
+  - particles with State == -1, which BGGS skips as targets and sources
+    (asymmetricSearch.f90:89-154, :223-226);
+  - a particle that leaves the fixed search grid (its key is clamped to the
+    border and its pairs take the FP64 test), while the reference rebuilds
+    its grid from the bounding box every step;
+  - an isolated particle with no neighbour, and a run with a single real
+    particle among the walls;
+  - a context with spare capacity, uploaded twice with different sets.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from mysph_amd import cases
+from mysph_amd import config as C
+from mysph_amd.sphgpu import Simulation
+from tests.parity import assert_close
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "rho", "p", "e")
+
+
+def _compare(case, cfg, steps, tol=1e-10, general=False, monkeypatch=None):
+    nt = case["n_real"]
+    if general:
+        monkeypatch.setenv("SPHGPU_FORCE_GENERAL", "1")
+    orc = oracle.OracleSim(case, cfg)
+    orc.run(steps)
+    with Simulation(case, cfg) as sim:
+        sim.advance(steps)
+        for k in FIELDS:
+            assert_close(k, sim.get(k)[:nt], orc[k][:nt], tol)
+        return sim.pairs(), orc
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_state_minus_one_particles_are_skipped(general, monkeypatch):
+    case = cases.dam_break_2d(dx=0.04)
+    rng = np.random.default_rng(11)
+    pick = rng.choice(case["n_real"], 40, replace=False)
+    case["state"][pick] = -1
+    cfg = case["config"].copy()
+    print(f"State -1 on 40 particles, general={general}", flush=True)
+    (off, nbr), _ = _compare(case, cfg, 30, general=general, monkeypatch=monkeypatch)
+    cnt = np.diff(off)
+    assert np.all(cnt[pick] == 0)                      # not targets
+    assert not np.isin(nbr, pick).any()                # not sources
+
+
+@pytest.mark.parametrize("precision", [C.FP64, C.MIXED])
+def test_particle_leaving_the_search_grid(precision):
+    case = cases.dam_break_2d(dx=0.04)
+    nt = case["n_real"]
+    # the top fluid particle is thrown straight up at 80 m/s: it leaves the
+    # tank over its walls and the grid planned at upload (walls to 3 m plus a
+    # margin) within a few hundred steps
+    i = int(np.argmax(case["x"][:nt, 1]))
+    case["v"][i] = (0.0, 80.0)
+    cfg = case["config"].copy()
+    cfg.precision = precision
+    steps = 300
+    print(f"escaping particle, precision={precision}", flush=True)
+    orc = oracle.OracleSim(case, cfg)
+    orc.run(steps)
+    with Simulation(case, cfg) as sim:
+        gx = sim.grid_info()
+        sim.advance(steps)
+        x = sim.get("x")
+        assert x[i, 1] > case["x"][:, 1].max() + 1.0    # far outside the initial box
+        tol = 1e-10 if precision == C.FP64 else None
+        if tol:
+            for k in FIELDS:
+                assert_close(k, sim.get(k)[:nt], orc[k][:nt], tol)
+        else:
+            # mixed: the same neighbour sets, and the state close to FP64's
+            assert_close("x", x[:nt], orc["x"][:nt], 1e-6)
+    assert gx is not None
+
+
+def test_isolated_and_single_particles(monkeypatch):
+    case = cases.dam_break_2d(dx=0.04)
+    nt = case["n_real"]
+    # one fluid particle far above the column: no neighbour at all
+    case["x"][nt - 1] = (0.5, 2.9)
+    (off, nbr), _ = _compare(case, case["config"].copy(), 20)
+    assert np.diff(off)[nt - 1] == 0
+    # a single real particle among the walls
+    one = dict(case)
+    keep = np.r_[[0], np.arange(nt, case["x"].shape[0])]
+    for k in ("x", "v", "mass", "rho", "p", "e", "c", "h", "type", "state", "boundary"):
+        one[k] = np.ascontiguousarray(case[k][keep])
+    one["n_real"] = 1
+    _compare(one, one["config"].copy(), 20)
+
+
+def test_spare_capacity_and_second_upload():
+    """A context keeps its config; a second upload replaces the particle set
+    (here a larger one) and restarts at step 1."""
+    a = cases.dam_break(2, 0.04, column=(0.8, 1.6))
+    b = cases.dam_break_2d(dx=0.04)
+    cfg = a["config"].copy()
+    cap = b["x"].shape[0] + 1000
+    ref = {}
+    for name, case in (("a", a), ("b", b)):
+        orc = oracle.OracleSim(case, cfg)
+        orc.run(25)
+        ref[name] = orc
+    with Simulation(a, cfg, capacity=cap) as sim:
+        sim.advance(25)
+        nt = a["n_real"]
+        for k in FIELDS:
+            assert_close("first " + k, sim.get(k)[:nt], ref["a"][k][:nt], 1e-10)
+        sim.n_real, sim.n_dummy = b["n_real"], b["n_dummy"]
+        sim.n = b["x"].shape[0]
+        sim.upload(b)
+        assert sim.step == 1
+        sim.advance(25)
+        nt = b["n_real"]
+        for k in FIELDS:
+            assert_close("second " + k, sim.get(k)[:nt], ref["b"][k][:nt], 1e-10)
