@@ -187,16 +187,15 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     R4 vi, thi;
     R pix = 0, piy = 0, piz = 0;
     int cxi = 0, cyi = 0, czi = 0;
-    bool esc_i = false;
+    uint32_t pk_i = 0;
     float lof = 0.0f, hif = 0.0f;
     if (MIXED) {
         vi = *(const R4*)&D.prf[s].vm;
         thi = *(const R4*)&D.prf[s].th;
         const float4 pf = D.posf[s];
         pix = pf.x; piy = pf.y; piz = pf.z;
-        const uint32_t pk = __float_as_uint(pf.w);
-        esc_i = pk >> 31;
-        unpack_cell<DIM>(pk, cxi, cyi, czi);
+        pk_i = __float_as_uint(pf.w);
+        unpack_cell<DIM>(pk_i, cxi, cyi, czi);
         // FP32 r^2 from cell offsets is within M of the FP64 one
         const double u = 5.9604644775390625e-8;
         const double e = 2.0 * u * P.cs + u * cut;
@@ -301,9 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
         return inwin;
     };
     if constexpr (MIXED) {
-        auto candidate = [&](int j, const float4& pf, const PairRecF& rj) {
-            const float4& vj = rj.vm;
-            const float4& thj = rj.th;
+        auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj, bool valid) {
             const uint32_t pk = __float_as_uint(pf.w);
             int cxj, cyj, czj;
             unpack_cell<DIM>(pk, cxj, cyj, czj);
@@ -311,14 +308,17 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             const R dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
             const R dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
             const R r2 = dx * dx + dy * dy + dz * dz;
+            // acceptance in bitwise logic (one mask, no nested regions); the
+            // FP64 test runs only where the FP32 distance is within its error
+            // bound of the cutoff or a particle is off the grid
+            const bool esc = (int)(pk | pk_i) < 0;   // bit 31: off the grid
+            bool ok = valid & ((r2 < hif) | esc);
+            if (ok & (!(r2 < lof) | esc)) ok = exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) >= 0.0;
+            if (need_win & ok) ok = in_window(j);
             // every candidate runs the pair arithmetic and a rejected one adds
-            // zeros (G = 0): no divergent branch around the sums
-            const bool esc = esc_i || (pk >> 31);
-            bool ok = (r2 < hif) || esc;
-            if (ok && (!(r2 < lof) || esc) && exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) ok = false;
-            if (ok && !in_window(j)) ok = false;
-            // vm.w = m / rho; one rsq gives r and 1 / r; both kernel
-            // branches are cheap, so select instead of diverging
+            // zeros (G = 0): no divergent branch around the sums. vm.w = m / rho;
+            // one rsq gives r and 1 / r; both kernel branches are cheap, so
+            // select instead of diverging
             const R rinv = fast_rsq(r2);
             const R r = r2 * rinv;
             const R q = r * inv_h;
@@ -329,28 +329,33 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             sum_pair(j, dx, dy, dz, r2, r, ok ? (q < (R)1 ? Gn : Gf) : (R)0, vj, thj, thj.w, vj.w,
                      vj.w * thj.x);
         };
-        // software pipeline: the three records of candidate k + 1 are in
-        // flight while candidate k is tested and summed (loaded whether or
-        // not it turns out to be a neighbour)
-        int j = n > 0 ? nl[0] : s;
-        int jn = n > 1 ? nl[P.ldn] : j;
-        float4 pf = D.posf[j];
-        PairRecF rj = D.prf[j];
-        for (int k = 0; k < n; ++k) {
-            const int jnn = k + 2 < n ? nl[(int64_t)(k + 2) * P.ldn] : jn;
-            const float4 pfn = D.posf[jn];
-            const PairRecF rjn = D.prf[jn];
-            candidate(j, pf, rj);
-            j = jn;
-            jn = jnn;
-            pf = pfn;
-            rj = rjn;
+        // software pipeline, unrolled by two so that the two record sets
+        // alternate instead of being copied: the records of candidate k + 1
+        // are in flight while candidate k is tested and summed (loaded whether
+        // or not it turns out to be a neighbour; indices past the end repeat
+        // the last valid one). The candidate list is read once per pass:
+        // nontemporal loads keep it from evicting neighbour records from L1/L2
+        const int64_t ldn = P.ldn;
+        int ja = n > 0 ? __builtin_nontemporal_load(nl) : s;
+        int jb = n > 1 ? __builtin_nontemporal_load(nl + ldn) : ja;
+        float4 pfa = D.posf[ja], vma = D.prf[ja].vm, tha = D.prf[ja].th;
+        for (int k = 0; k < n; k += 2) {
+            const float4 pfb = D.posf[jb], vmb = D.prf[jb].vm, thb = D.prf[jb].th;
+            const int jc = k + 2 < n ? __builtin_nontemporal_load(nl + (int64_t)(k + 2) * ldn) : jb;
+            candidate(ja, pfa, vma, tha, true);
+            pfa = D.posf[jc];
+            vma = D.prf[jc].vm;
+            tha = D.prf[jc].th;
+            const int jd = k + 3 < n ? __builtin_nontemporal_load(nl + (int64_t)(k + 3) * ldn) : jc;
+            candidate(jb, pfb, vmb, thb, k + 1 < n);
+            ja = jc;
+            jb = jd;
         }
     } else {
-        int jn = n > 0 ? nl[0] : 0;
+        int jn = n > 0 ? __builtin_nontemporal_load(nl) : 0;
         for (int k = 0; k < n; ++k) {
             const int j = jn;
-            if (k + 1 < n) jn = nl[(int64_t)(k + 1) * P.ldn];
+            if (k + 1 < n) jn = __builtin_nontemporal_load(nl + (int64_t)(k + 1) * P.ldn);
             const double4 qj = D.posh[j];
             const double dr = exact_dr<DIM>(qi, qj, cut, lo2, hi2);
             if (dr < 0.0) continue;
