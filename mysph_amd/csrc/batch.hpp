@@ -227,7 +227,7 @@ extern "C" int sphgpu_batch_create(int32_t dim, int32_t precision, int64_t slot_
     rc |= balloc(b, &D.key, N); rc |= balloc(b, &D.refcell, N); rc |= balloc(b, &D.perm, N);
     rc |= balloc(b, &D.cpos, N); rc |= balloc(b, &D.vcnt, N);
     rc |= balloc(b, &D.ecnt, N);
-    rc |= balloc(b, &D.vnbr, (size_t)N * b->verlet_cap);
+    rc |= balloc(b, &D.vnbr, (size_t)vlist_entries(N, b->verlet_cap));
     if (precision == SPHGPU_MIXED) {
         rc |= balloc(b, &D.dispf, N); rc |= balloc(b, &D.xbf, N);
         rc |= balloc(b, &D.ef, N); rc |= balloc(b, &D.eprevf, N); rc |= balloc(b, &D.divf, N);

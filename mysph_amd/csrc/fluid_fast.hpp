@@ -11,6 +11,22 @@
 // the reference's arithmetic (asymmetricSearch.f90:227-237) and window
 // (:209-217).
 
+// Candidate-list layout: each group of kVlistLanes consecutive slots (one
+// wave) owns a contiguous block of cap rows of kVlistLanes entries; entry k of
+// slot s is at row k of its group's block. A wave's load of entry k is one
+// contiguous 256-byte row, and its whole list is one contiguous stream, so
+// HBM reads it page by page instead of one row per DRAM page.
+constexpr int kVlistLanes = 64;
+
+__host__ __device__ __forceinline__ int64_t vlist_at(int s, int k, int cap) {
+    return ((int64_t)(s / kVlistLanes) * cap + k) * kVlistLanes + s % kVlistLanes;
+}
+
+// entries of a candidate list for `slots` slots with `cap` rows each
+inline int64_t vlist_entries(int64_t slots, int cap) {
+    return (slots + kVlistLanes - 1) / kVlistLanes * kVlistLanes * cap;
+}
+
 // Predict + EOS + pair inputs for every particle, the reference cell (for the
 // window check), and the largest displacement since the last list build.
 template <int DIM, bool MIXED, bool BATCH>
@@ -99,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
             const float r2 = dx * dx + dy * dy + dz * dz;
             const int j = __float_as_int(cj.w);
             if (r2 < lim && j != s) {
-                if (cnt < cap) D.vnbr[(int64_t)cnt * P.ldn + s] = j;
+                if (cnt < cap) D.vnbr[vlist_at(s, cnt, cap)] = j;
                 ++cnt;
             }
         };
@@ -229,40 +245,57 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
     int acc_n = 0;
     const int n = (mt.y == -1) ? 0 : min(D.vcnt[s], P.verlet_cap);
-    const int* __restrict__ nl = D.vnbr + s;
+    const int* __restrict__ nl = D.vnbr + vlist_at(s, 0, P.verlet_cap);
+    // the mixed pair sums in fewer operations (the same quantities, FP32
+    // rounding): grad W = G dx, so dv . grad W = G (dx . dv) and
+    // dx . grad W = G r^2; the energy rate 1/2 sum m_j (aux + Pi) dv . grad W
+    // is 1/2 sum cf dv . grad W; and with A = (h_i + h_j) dx . dv and
+    // B = r^2 + psi^2 hbar^2, Pi = (beta A / 2 - alpha (c_i + c_j) B / 2) A /
+    // (B^2 (rho_i + rho_j)): one reciprocal instead of two
+    const R beta_h = (R)0.5 * beta, alpha_h = (R)0.5 * alpha, psi2_q = (R)0.25 * psi2;
     // physics of one accepted pair, both precisions
     auto sum_pair = [&](int j, R dx, R dy, R dz, R r2, R r, R G, const R4& vj, const R4& thj, R hj, R Vj,
                         R mj) {
-        if constexpr (!MIXED) ++acc_n;
-        const R gx = G * dx, gy = G * dy, gz = G * dz;
         const R dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = DIM == 3 ? vi.z - vj.z : (R)0;
-        const R dvg = dvx * gx + dvy * gy + dvz * gz;
-        const R dxg = dx * gx + dy * gy + dz * gz;
-        sv += Vj * dvg;
-        sx += Vj * dxg;
-        const R aux = thi.y + thj.y;
         const R xv = dx * dvx + dy * dvy + dz * dvz;
-        const R hij = (R)0.5 * (hi + hj);
-        R Pi;
+        const R aux = thi.y + thj.y;
         if constexpr (MIXED) {
+            const R dvg = G * xv;
+            sv += Vj * dvg;
+            sx += Vj * (G * r2);
             // evaluated for every pair and selected: a branch here lets the
             // compiler split the neighbour record load and wait twice
-            const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
-            const R piv = (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
-                          fast_rcp((R)0.5 * (thi.x + thj.x));
-            Pi = xv < (R)0 ? piv : (R)0;
+            const R hs = hi + hj;
+            const R A = hs * xv;
+            const R B = psi2_q * hs * hs + r2;
+            const R num = beta_h * A - alpha_h * (thi.z + thj.z) * B;
+            const R piv = num * A * fast_rcp(B * B * (thi.x + thj.x));
+            const R Pi = xv < (R)0 ? piv : (R)0;
+            const R cf = mj * (aux + Pi);
+            de_in += cf * dvg;
+            const R cfg = cf * G;
+            ax -= cfg * dx;
+            ay -= cfg * dy;
+            az -= cfg * dz;
         } else {
+            ++acc_n;
+            const R gx = G * dx, gy = G * dy, gz = G * dz;
+            const R dvg = dvx * gx + dvy * gy + dvz * gz;
+            const R dxg = dx * gx + dy * gy + dz * gz;
+            sv += Vj * dvg;
+            sx += Vj * dxg;
+            const R hij = (R)0.5 * (hi + hj);
             const R phi = hij * xv * fast_rcp(r2 + psi2 * hij * hij);
-            Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
-                                 fast_rcp((R)0.5 * (thi.x + thj.x))
-                           : (R)0;
+            const R Pi = xv < (R)0 ? (beta * phi - alpha * (R)0.5 * (thi.z + thj.z)) * phi *
+                                         fast_rcp((R)0.5 * (thi.x + thj.x))
+                                   : (R)0;
+            const R cf = mj * (aux + Pi);
+            ax -= cf * gx;
+            ay -= cf * gy;
+            az -= cf * gz;
+            de_in += mj * aux * dvg;
+            de_av += (R)0.5 * mj * Pi * dvg;
         }
-        const R cf = mj * (aux + Pi);
-        ax -= cf * gx;
-        ay -= cf * gy;
-        az -= cf * gz;
-        de_in += mj * aux * dvg;
-        de_av += (R)0.5 * mj * Pi * dvg;
         if (PAIR_EXT) {
             const int tj = D.meta[j].x;
             R fe = 0;
@@ -300,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
         return inwin;
     };
     if constexpr (MIXED) {
-        auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj, bool valid) {
+        auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj) {
             const uint32_t pk = __float_as_uint(pf.w);
             int cxj, cyj, czj;
             unpack_cell<DIM>(pk, cxj, cyj, czj);
@@ -310,10 +343,11 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             const R r2 = dx * dx + dy * dy + dz * dz;
             // acceptance in bitwise logic (one mask, no nested regions); the
             // FP64 test runs only where the FP32 distance is within its error
-            // bound of the cutoff or a particle is off the grid
-            const bool esc = (int)(pk | pk_i) < 0;   // bit 31: off the grid
-            bool ok = valid & ((r2 < hif) | esc);
-            if (ok & (!(r2 < lof) | esc)) ok = exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) >= 0.0;
+            // bound of the cutoff or a particle is off the grid (bit 31 of the
+            // cell word): such a pair is tested at r2 = lof, inside the band
+            const R r2t = (int)(pk | pk_i) < 0 ? lof : r2;
+            bool ok = r2t < hif;
+            if (ok & !(r2t < lof)) ok = exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) >= 0.0;
             if (need_win & ok) ok = in_window(j);
             // every candidate runs the pair arithmetic and a rejected one adds
             // zeros (G = 0): no divergent branch around the sums. vm.w = m / rho;
@@ -329,33 +363,31 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             sum_pair(j, dx, dy, dz, r2, r, ok ? (q < (R)1 ? Gn : Gf) : (R)0, vj, thj, thj.w, vj.w,
                      vj.w * thj.x);
         };
-        // software pipeline, unrolled by two so that the two record sets
-        // alternate instead of being copied: the records of candidate k + 1
-        // are in flight while candidate k is tested and summed (loaded whether
-        // or not it turns out to be a neighbour; indices past the end repeat
-        // the last valid one). The candidate list is read once per pass:
-        // nontemporal loads keep it from evicting neighbour records from L1/L2
-        const int64_t ldn = P.ldn;
-        int ja = n > 0 ? __builtin_nontemporal_load(nl) : s;
-        int jb = n > 1 ? __builtin_nontemporal_load(nl + ldn) : ja;
-        float4 pfa = D.posf[ja], vma = D.prf[ja].vm, tha = D.prf[ja].th;
-        for (int k = 0; k < n; k += 2) {
-            const float4 pfb = D.posf[jb], vmb = D.prf[jb].vm, thb = D.prf[jb].th;
-            const int jc = k + 2 < n ? __builtin_nontemporal_load(nl + (int64_t)(k + 2) * ldn) : jb;
-            candidate(ja, pfa, vma, tha, true);
-            pfa = D.posf[jc];
-            vma = D.prf[jc].vm;
-            tha = D.prf[jc].th;
-            const int jd = k + 3 < n ? __builtin_nontemporal_load(nl + (int64_t)(k + 3) * ldn) : jc;
-            candidate(jb, pfb, vmb, thb, k + 1 < n);
-            ja = jc;
-            jb = jd;
+        // software pipeline: the records of candidate k + 1 are in flight
+        // while candidate k is tested and summed (loaded whether or not it
+        // turns out to be a neighbour; indices past the end repeat the last
+        // valid one). The candidate list is read once per pass: nontemporal
+        // loads keep it from evicting neighbour records from L1/L2. Unsigned
+        // slot indices make each record address one shift-add.
+        constexpr int ldn = kVlistLanes;
+        int j = n > 0 ? __builtin_nontemporal_load(nl) : s;
+        int jn = n > 1 ? __builtin_nontemporal_load(nl + ldn) : j;
+        float4 pf = D.posf[(uint32_t)j], vm = D.prf[(uint32_t)j].vm, th = D.prf[(uint32_t)j].th;
+        for (int k = 0; k < n; ++k) {
+            const int jnn = k + 2 < n ? __builtin_nontemporal_load(nl + (int64_t)(k + 2) * ldn) : jn;
+            const float4 pfn = D.posf[(uint32_t)jn], vmn = D.prf[(uint32_t)jn].vm, thn = D.prf[(uint32_t)jn].th;
+            candidate(j, pf, vm, th);
+            j = jn;
+            jn = jnn;
+            pf = pfn;
+            vm = vmn;
+            th = thn;
         }
     } else {
         int jn = n > 0 ? __builtin_nontemporal_load(nl) : 0;
         for (int k = 0; k < n; ++k) {
             const int j = jn;
-            if (k + 1 < n) jn = __builtin_nontemporal_load(nl + (int64_t)(k + 1) * P.ldn);
+            if (k + 1 < n) jn = __builtin_nontemporal_load(nl + (int64_t)(k + 1) * kVlistLanes);
             const double4 qj = D.posh[j];
             const double dr = exact_dr<DIM>(qi, qj, cut, lo2, hi2);
             if (dr < 0.0) continue;
@@ -376,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     }
     if (mt.z == 1) ax = ay = az = 0;
     const R drho = thi.x * sv;
-    const R de = (R)0.5 * de_in + de_av;
+    const R de = (R)0.5 * de_in + de_av;   // mixed: de_in holds both terms, de_av is 0
     if (MIXED) {
         D.dvf[s] = make_float4((float)ax, (float)ay, (float)az, (float)drho);
         D.def[s] = (float)de;

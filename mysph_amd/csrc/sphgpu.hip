@@ -107,7 +107,7 @@ struct Dev {
     float* ef;        // internal energy (FP64 e otherwise)
     float* eprevf;    // its value at the last correct (FP64 eprev otherwise)
     float2* divf;     // divergences of position and velocity (FP64 div otherwise)
-    int* vnbr;        // candidate list, column-major like nbr
+    int* vnbr;        // candidate list, one contiguous block per wave of slots (vlist_at)
     int* vcnt;
     int* ecnt;        // accepted neighbours per particle in the last force pass
     unsigned long long* vflag;   // rebuild gate (see Gate)
@@ -2112,9 +2112,10 @@ static int size_neighbour_list(sphgpu_ctx* c) {
         int vx = c->h_err[1];
         int vcap = vx + std::max(8, vx / 2);
         c->P.verlet_cap = vcap;
-        if ((int64_t)vcap * c->capacity > c->vnbr_alloc) {
-            if (dalloc(c, &c->D.vnbr, (size_t)vcap * c->capacity)) return SPHGPU_E_HIP;
-            c->vnbr_alloc = (int64_t)vcap * c->capacity;
+        const int64_t need = vlist_entries(c->capacity, vcap);
+        if (need > c->vnbr_alloc) {
+            if (dalloc(c, &c->D.vnbr, (size_t)need)) return SPHGPU_E_HIP;
+            c->vnbr_alloc = need;
         }
     }
     return 0;
