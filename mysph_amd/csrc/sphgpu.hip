@@ -225,24 +225,33 @@ __global__ __launch_bounds__(kBlock) void k_bbox(Params P0, Dev D, RunMap M) {
 template <int DIM, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_grid(Params P0, Dev D, RunMap M) {
     const Params& P = BATCH ? M.runs[M.rows[blockIdx.x]] : P0;
-    __shared__ double smn[3][kBlock], smx[3][kBlock];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ double wmn[3][kWaves], wmx[3][kWaves];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int b = threadIdx.x; b < P.nblk; b += kBlock)
         for (int d = 0; d < 3; ++d) {
             mn[d] = fmin(mn[d], P.partial[b * 6 + d]);
             mx[d] = fmax(mx[d], P.partial[b * 6 + 3 + d]);
         }
-    for (int d = 0; d < 3; ++d) { smn[d][threadIdx.x] = mn[d]; smx[d][threadIdx.x] = mx[d]; }
+    // min / max are order-independent: wave shuffles, then the block's waves
+    for (int d = 0; d < 3; ++d)
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[d] = fmin(mn[d], __shfl_xor(mn[d], o));
+            mx[d] = fmax(mx[d], __shfl_xor(mx[d], o));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 3; ++d) { wmn[d][threadIdx.x >> 6] = mn[d]; wmx[d][threadIdx.x >> 6] = mx[d]; }
     __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o)
-            for (int d = 0; d < 3; ++d) {
-                smn[d][threadIdx.x] = fmin(smn[d][threadIdx.x], smn[d][threadIdx.x + o]);
-                smx[d][threadIdx.x] = fmax(smx[d][threadIdx.x], smx[d][threadIdx.x + o]);
-            }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
+        double smn[3][1], smx[3][1];
+        for (int d = 0; d < 3; ++d) {
+            smn[d][0] = wmn[d][0];
+            smx[d][0] = wmx[d][0];
+            for (int w = 1; w < kWaves; ++w) {
+                smn[d][0] = fmin(smn[d][0], wmn[d][w]);
+                smx[d][0] = fmax(smx[d][0], wmx[d][w]);
+            }
+        }
         RefGrid g;
         double delta[3] = {0.0, 0.0, 0.0};
         for (int d = 0; d < 3; ++d) { g.minc[d] = 0.0; g.maxc[d] = 0.0; g.num[d] = 1; }
@@ -299,7 +308,13 @@ __global__ __launch_bounds__(kBlock) void k_grid(Params P0, Dev D, RunMap M) {
 // Our cell key (clamped into the fixed grid) and the reference cell.
 template <int DIM, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int with_ref, RunMap M) {
-    if (!gate_open(G)) return;
+    // with_ref 1: reference cells with the keys; 2 (one run): reference cells
+    // whenever the reference window can exclude pairs, keys only when the
+    // gate is open (the fast path's per-step reference cells ride on this
+    // launch instead of one of their own)
+    const bool keys_on = gate_open(G);
+    const bool ref_on = with_ref == 1 || (!BATCH && with_ref == 2 && P0.grid->need_win);
+    if (!keys_on && !ref_on) return;
     const int lim = BATCH ? M.nslot : P0.n;
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
     const Params* Pp = slot_run<BATCH>(P0, M, s);
@@ -321,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int w
         key = (uint32_t)((int64_t)c[0] + (int64_t)P.nc[0] * ((int64_t)c[1] + (int64_t)P.nc[1] * c[2]));
     }
     key += (uint32_t)P.c0;
-    if (with_ref && mt.y != -1) {
+    if (ref_on && mt.y != -1) {
         double xs[3] = {q.x, q.y, q.z};
         const RefGrid g = *P.grid;
         bool bad = false;
@@ -331,8 +346,10 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int w
                                                                                   : kErrNonFinite);
         D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
     }
-    D.key[s] = key;
-    atomicAdd(&D.count[key], 1);
+    if (keys_on) {
+        D.key[s] = key;
+        atomicAdd(&D.count[key], 1);
+    }
     }
 }
 
@@ -1781,7 +1798,7 @@ static void launch_fast_tail(sphgpu_ctx* c, int count_only) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const Gate G = fast_gate(c);
-    launch_sort<DIM>(c, G, 0);
+    launch_sort<DIM>(c, G, 2);
     hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only, RunMap{});
     if (count_only) return;
     if (c->time_force) hipEventRecord(c->ev[2], c->stream);
@@ -1809,7 +1826,6 @@ template <int DIM>
 static void fast_body_step(sphgpu_ctx* c, bool timed) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
-    const int nbs = std::min(nb, kStrideBlocks);
     c->time_force = timed;
     launch_fast_tail<DIM>(c, 0);
     const int first = c->step == 1;
@@ -1818,7 +1834,6 @@ static void fast_body_step(sphgpu_ctx* c, bool timed) {
     else
         hipLaunchKernelGGL((k_correct_predict<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
     hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
-    hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
     c->time_force = false;
 }
 
