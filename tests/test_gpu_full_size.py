@@ -10,6 +10,9 @@ this case for many steps in test time:
     (their count, and the largest, smallest and zero per-target counts) and
     the exported exact list (the sorted pair set); the search grids agree
     exactly;
+  - FP64 against the oracle: step-1 derivatives within 1e-12 and the state
+    after 10 leap-frog steps within 1e-10 (the oracle takes about a second
+    per step here);
   - a run repeated in a fresh context is bit-identical;
   - mixed precision stays within the density bound of the FP64 run
     (1e-4 relative; about 1e-8 after 200 steps).
@@ -25,7 +28,7 @@ import oracle
 from mysph_amd import cases
 from mysph_amd import config as C
 from mysph_amd.sphgpu import Simulation
-from tests.parity import rel_err, sorted_pairs
+from tests.parity import assert_close, rel_err, sorted_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -88,3 +91,22 @@ def test_full_size_repeat_and_mixed_bound(case):
     print(f"mixed vs FP64 after {STEPS} steps: rho {e_rho:.2e}, x {e_x:.2e}", flush=True)
     assert e_rho <= 1e-4
     assert e_x <= 1e-6
+
+
+def test_full_size_fp64_against_oracle(case):
+    nt = case["n_real"]
+    print("full size: step-1 derivatives and 10 steps against the oracle", flush=True)
+    orc = oracle.OracleSim(case)
+    orc.derivatives()
+    with Simulation(case) as sim:
+        sim.derivatives()
+        for k in ("dvdt", "drhodt", "dedt"):
+            e = assert_close("step-1 " + k, sim.get(k)[:nt], orc[k][:nt], 1e-12)
+            print(f"step-1 {k}: {e:.2e}", flush=True)
+    orc = oracle.OracleSim(case)
+    orc.run(10)
+    with Simulation(case) as sim:
+        sim.advance(10)
+        for k in ("x", "v", "rho", "p", "e"):
+            e = assert_close("10 steps " + k, sim.get(k)[:nt], orc[k][:nt], 1e-10)
+            print(f"10 steps {k}: {e:.2e}", flush=True)
