@@ -1995,7 +1995,11 @@ static int check_err(sphgpu_ctx* c) {
         HIPCHK(c, hipMemsetAsync(c->D.err, 0, sizeof(int), c->stream));
         if (e & kErrNonFinite) return fail(c, SPHGPU_E_NONFINITE, "non-finite position, velocity or density");
         if (e & kErrOutOfRange) return fail(c, SPHGPU_E_OUT_OF_RANGE, "particle out of range");
-        if (e & kErrPairCap) return fail(c, SPHGPU_E_PAIR_CAPACITY, "neighbour list capacity exceeded");
+        if (e & kErrPairCap)
+            return fail(c, SPHGPU_E_PAIR_CAPACITY,
+                        "neighbour list capacity exceeded: the lists are planned at upload for 1.5x its "
+                        "largest neighbour count + 8, and this step's pair sums are incomplete "
+                        "(upload the last good state again)");
         if (e & kErrHRange)
             return fail(c, SPHGPU_E_OUT_OF_RANGE,
                         "smoothing length grew past the planned search range (upload the state again)");

@@ -9,7 +9,10 @@ apply:
     its grid from the bounding box every step;
   - an isolated particle with no neighbour, and a run with a single real
     particle among the walls;
-  - a context with spare capacity, uploaded twice with different sets.
+  - a context with spare capacity, uploaded twice with different sets;
+  - a particle set that outgrows the neighbour lists planned at upload: the
+    step fails with SPHGPU_E_PAIR_CAPACITY (the reference grows its list,
+    asymmetricSearch.f90:239-245), and the state uploads again.
 """
 from __future__ import annotations
 
@@ -19,7 +22,7 @@ import pytest
 import oracle
 from mysph_amd import cases
 from mysph_amd import config as C
-from mysph_amd.sphgpu import Simulation
+from mysph_amd.sphgpu import Simulation, SphGpuError
 from tests.parity import assert_close
 
 pytestmark = pytest.mark.gpu
@@ -125,3 +128,44 @@ def test_spare_capacity_and_second_upload():
         nt = b["n_real"]
         for k in FIELDS:
             assert_close("second " + k, sim.get(k)[:nt], ref["b"][k][:nt], 1e-10)
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_outgrown_lists_fail_loudly(general, monkeypatch):
+    """A 20 x 20 cloud at twice the lattice spacing (4 neighbours each)
+    converges ballistically (no EOS, viscosity or gravity: v = -20 (x - c)).
+    Near half its spacing every particle has 20 neighbours, more than the
+    12 the lists were planned for, and the step fails instead of dropping
+    pairs. Uploading the state again plans larger lists."""
+    if general:
+        monkeypatch.setenv("SPHGPU_FORCE_GENERAL", "1")
+    dx, m = 0.02, 20
+    g = np.arange(m) * 2.0 * dx
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    x = np.stack([X.ravel(), Y.ravel()], axis=1) + 1.0
+    n = x.shape[0]
+    v = -20.0 * (x - x.mean(axis=0))
+    cfg = C.dam_break_config(2, 1.0, 1.2 * dx)
+    cfg.gravity = 0
+    cfg.artificial_viscosity = 0
+    cfg.mat[0].eos = C.EOS_NONE          # pressure stays at its uploaded 0
+    case = dict(x=x, v=v, mass=np.full(n, 1000.0 * dx * dx), rho=np.full(n, 1000.0),
+                p=np.zeros(n), e=np.zeros(n), c=np.zeros(n), h=np.full(n, 1.2 * dx),
+                type=np.full(n, 2, dtype=np.int32), state=np.zeros(n, dtype=np.int32),
+                boundary=np.zeros(n, dtype=np.int32), n_real=n, n_dummy=0, dim=2, config=cfg)
+    print(f"outgrown lists, general={general}", flush=True)
+    err = None
+    with Simulation(case, cfg) as sim:
+        for _ in range(20):
+            try:
+                sim.advance(20)
+            except SphGpuError as e:
+                err = e
+                break
+        assert err is not None, "the compressed cloud never outgrew its lists"
+        assert err.code == -3 and "upload" in str(err), str(err)
+        state = {k: sim.get(k) for k in ("x", "v", "rho")}
+    again = dict(case, **state)
+    with Simulation(again, cfg) as sim:
+        sim.advance(10)
+        assert np.all(np.isfinite(sim.get("x")))
