@@ -40,11 +40,14 @@ typedef enum sphgpu_status {
     SPHGPU_OK = 0,
     SPHGPU_E_OUT_OF_RANGE = -1,   /* particle outside the grid (asymmetricSearch.f90:38-39) */
     SPHGPU_E_NONFINITE = -2,      /* NaN/Inf position, density or velocity */
-    SPHGPU_E_PAIR_CAPACITY = -3,  /* pair export buffer too small; or, from a step, a
-                                   * particle outgrew the neighbour lists planned at
-                                   * upload (1.5x the largest count + 8; the reference
-                                   * grows its list instead): that step's sums are
-                                   * incomplete, upload the last good state again */
+    SPHGPU_E_PAIR_CAPACITY = -3,  /* pair export buffer too small; or, from a step on
+                                   * the general path or in a batch, a particle outgrew
+                                   * the neighbour lists planned at upload (1.5x the
+                                   * largest count + 8; the reference grows its list):
+                                   * that step's sums are incomplete, upload the last
+                                   * good state again. A single run's fluid fast path
+                                   * walks the cell window of such a particle instead
+                                   * and stays exact. */
     SPHGPU_E_BAD_CONFIG = -4,     /* unsupported switch combination */
     SPHGPU_E_HIP = -5,            /* HIP runtime error */
     SPHGPU_E_ARG = -6,            /* bad argument (NULL, size) */

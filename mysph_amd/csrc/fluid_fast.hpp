@@ -144,7 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
             y = ny; z = nz; kb = nkb; ke = nke;
         }
         D.vcnt[s] = cnt;
-        if (!count_only && cnt > P.verlet_cap) atomicOr(P.err, kErrPairCap);
+        // one run's force pass walks the cell window of a particle whose list
+        // overflowed; a batch's cannot (see k_force_fused)
+        if (BATCH && !count_only && cnt > P.verlet_cap) atomicOr(P.err, kErrPairCap);
     }
     // one atomic per wave for the largest count (same-address atomics from
     // every lane serialise in L2)
@@ -244,7 +246,15 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, psi2 = (R)(P.av_psi * P.av_psi);
     R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
     int acc_n = 0;
-    const int n = (mt.y == -1) ? 0 : min(D.vcnt[s], P.verlet_cap);
+    // a particle with more candidates than its list holds (one run) walks its
+    // cell window of the last sort instead: that window holds every
+    // candidate the list would (the sort is at or after the list's build),
+    // and the exact test keeps only the reference's pairs. A batch's shared
+    // cell tables do not hold a run that is not rebuilding, so there the
+    // list build reports the overflow instead.
+    const int nc = (mt.y == -1) ? 0 : D.vcnt[s];
+    const bool sweep = !BATCH && nc > P.verlet_cap;
+    const int n = sweep ? 0 : min(nc, P.verlet_cap);
     const int* __restrict__ nl = D.vnbr + vlist_at(s, 0, P.verlet_cap);
     // the mixed pair sums in fewer operations (the same quantities, FP32
     // rounding): grad W = G dx, so dv . grad W = G (dx . dv) and
@@ -332,6 +342,29 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
         for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
         return inwin;
     };
+    // every particle but s in the +-win cell window around s's cell of the
+    // last sort, in cell order (the order of the list build)
+    auto for_window = [&](auto&& f) {
+        const uint32_t key = D.key[s] - (uint32_t)P.c0;
+        const int* __restrict__ start = D.start + P.c0;
+        const int cx = (int)(key % (uint32_t)P.nc[0]);
+        const int rest = (int)(key / (uint32_t)P.nc[0]);
+        const int cy = rest % P.nc[1];
+        const int cz = rest / P.nc[1];
+        const int W = P.win;
+        const int zlo = DIM == 3 ? max(cz - W, 0) : 0, zhi = DIM == 3 ? min(cz + W, P.nc[2] - 1) : 0;
+        const int ylo = max(cy - W, 0), yhi = min(cy + W, P.nc[1] - 1);
+        const int xlo = max(cx - W, 0), xhi = min(cx + W, P.nc[0] - 1);
+        for (int z = zlo; z <= zhi; ++z)
+            for (int y = ylo; y <= yhi; ++y) {
+                const int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
+                const int ke = start[row + xhi + 1];
+                for (int k = start[row + xlo]; k < ke; ++k) {
+                    const int j = __float_as_int(D.cpos[k].w);
+                    if (j != s) f(j);
+                }
+            }
+    };
     if constexpr (MIXED) {
         auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj) {
             const uint32_t pk = __float_as_uint(pf.w);
@@ -383,15 +416,16 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             vm = vmn;
             th = thn;
         }
+        if (sweep)
+            for_window([&](int j) {
+                candidate(j, D.posf[(uint32_t)j], D.prf[(uint32_t)j].vm, D.prf[(uint32_t)j].th);
+            });
     } else {
-        int jn = n > 0 ? __builtin_nontemporal_load(nl) : 0;
-        for (int k = 0; k < n; ++k) {
-            const int j = jn;
-            if (k + 1 < n) jn = __builtin_nontemporal_load(nl + (int64_t)(k + 1) * kVlistLanes);
+        auto candidate64 = [&](int j) {
             const double4 qj = D.posh[j];
             const double dr = exact_dr<DIM>(qi, qj, cut, lo2, hi2);
-            if (dr < 0.0) continue;
-            if (!in_window(j)) continue;
+            if (dr < 0.0) return;
+            if (!in_window(j)) return;
             const R4 vj = *(const R4*)&D.vm[j];
             const R4 thj = *(const R4*)&D.th[j];
             const R r = fast_sqrt((R)dr);
@@ -400,7 +434,14 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             const R G = q < (R)1 ? g1 * ((R)-2 + (R)1.5 * q) : g2 * t * t * fast_rcp(r);
             sum_pair(j, (R)(qi.x - qj.x), (R)(qi.y - qj.y), DIM == 3 ? (R)(qi.z - qj.z) : (R)0, (R)dr, r, G,
                      vj, thj, (R)qj.w, thj.w, vj.w);
+        };
+        int jn = n > 0 ? __builtin_nontemporal_load(nl) : 0;
+        for (int k = 0; k < n; ++k) {
+            const int j = jn;
+            if (k + 1 < n) jn = __builtin_nontemporal_load(nl + (int64_t)(k + 1) * kVlistLanes);
+            candidate64(j);
         }
+        if (sweep) for_window(candidate64);
     }
     if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100)) {
         if (DIM == 2) ay += (R)P.g;

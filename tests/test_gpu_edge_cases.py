@@ -11,8 +11,9 @@ apply:
     particle among the walls;
   - a context with spare capacity, uploaded twice with different sets;
   - a particle set that outgrows the neighbour lists planned at upload: the
-    step fails with SPHGPU_E_PAIR_CAPACITY (the reference grows its list,
-    asymmetricSearch.f90:239-245), and the state uploads again.
+    fast path walks the cell window of such a particle and stays exact; the
+    general path fails with SPHGPU_E_PAIR_CAPACITY (the reference grows its
+    list, asymmetricSearch.f90:239-245), and the state uploads again.
 """
 from __future__ import annotations
 
@@ -130,15 +131,12 @@ def test_spare_capacity_and_second_upload():
             assert_close("second " + k, sim.get(k)[:nt], ref["b"][k][:nt], 1e-10)
 
 
-@pytest.mark.parametrize("general", [False, True])
-def test_outgrown_lists_fail_loudly(general, monkeypatch):
-    """A 20 x 20 cloud at twice the lattice spacing (4 neighbours each)
-    converges ballistically (no EOS, viscosity or gravity: v = -20 (x - c)).
-    Near half its spacing every particle has 20 neighbours, more than the
-    12 the lists were planned for, and the step fails instead of dropping
-    pairs. Uploading the state again plans larger lists."""
-    if general:
-        monkeypatch.setenv("SPHGPU_FORCE_GENERAL", "1")
+def _converging_cloud():
+    """A 20 x 20 cloud at twice the lattice spacing (4 neighbours each) that
+    converges ballistically: no EOS (pressure stays at its uploaded 0), no
+    viscosity, no gravity, v = -20 (x - c). Near half its spacing every
+    particle has about 20 neighbours, more than the 12 the lists are planned
+    for at upload (1.5x the largest count + 8)."""
     dx, m = 0.02, 20
     g = np.arange(m) * 2.0 * dx
     X, Y = np.meshgrid(g, g, indexing="ij")
@@ -148,12 +146,37 @@ def test_outgrown_lists_fail_loudly(general, monkeypatch):
     cfg = C.dam_break_config(2, 1.0, 1.2 * dx)
     cfg.gravity = 0
     cfg.artificial_viscosity = 0
-    cfg.mat[0].eos = C.EOS_NONE          # pressure stays at its uploaded 0
-    case = dict(x=x, v=v, mass=np.full(n, 1000.0 * dx * dx), rho=np.full(n, 1000.0),
+    cfg.mat[0].eos = C.EOS_NONE
+    return dict(x=x, v=v, mass=np.full(n, 1000.0 * dx * dx), rho=np.full(n, 1000.0),
                 p=np.zeros(n), e=np.zeros(n), c=np.zeros(n), h=np.full(n, 1.2 * dx),
                 type=np.full(n, 2, dtype=np.int32), state=np.zeros(n, dtype=np.int32),
                 boundary=np.zeros(n, dtype=np.int32), n_real=n, n_dummy=0, dim=2, config=cfg)
-    print(f"outgrown lists, general={general}", flush=True)
+
+
+def test_outgrown_candidate_lists_stay_exact():
+    """Fast path: a particle whose candidates outgrow its list walks its cell
+    window instead, so the steps stay exact (the reference grows its list,
+    asymmetricSearch.f90:239-245)."""
+    case = _converging_cloud()
+    nt = case["n_real"]
+    print("outgrown candidate lists, fast path", flush=True)
+    orc = oracle.OracleSim(case)
+    orc.run(300)
+    with Simulation(case) as sim:
+        sim.advance(300)
+        assert sim.neighbour_stats()["max"] > 12     # past the planned list
+        for k in FIELDS:
+            assert_close(k, sim.get(k)[:nt], orc[k][:nt], 1e-10)
+
+
+def test_outgrown_lists_fail_loudly(monkeypatch):
+    """General path: the exact list planned at upload is outgrown, and the
+    step fails with SPHGPU_E_PAIR_CAPACITY instead of dropping pairs.
+    Uploading the state again plans larger lists."""
+    monkeypatch.setenv("SPHGPU_FORCE_GENERAL", "1")
+    case = _converging_cloud()
+    cfg = case["config"]
+    print("outgrown lists, general path", flush=True)
     err = None
     with Simulation(case, cfg) as sim:
         for _ in range(20):
