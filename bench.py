@@ -90,20 +90,43 @@ def run_gpu(case, precision, warmup, steps):
     t1 = time.perf_counter()
     adv_ms, force_ms, _ = sim.timing()
     stats = sim.neighbour_stats()
-    rho = sim.get("rho")[: case["n_real"]]
+    state = {k: sim.get(k)[: case["n_real"]] for k in ("rho", "x", "v")}
     sim.close()
     return dict(wall_s=t1 - t0, event_ms=adv_ms, force_ms=force_ms, pairs=stats["pairs"],
-                max_nbr=stats["max"], rho=rho, label=C.FP64 if precision == 0 else C.MIXED)
+                max_nbr=stats["max"], rho=state["rho"], state=state,
+                label=C.FP64 if precision == 0 else C.MIXED)
 
 
-def accuracy(case, nsteps):
+def force_samples(case, precision, warmup, steps, every=10):
+    """Average duration and pair count of the force pass over the timed
+    steps. The timed run replays captured graphs, in which no launch can be
+    timed alone, so a second run of the same workload advances in calls of
+    `every` steps and times each call's closing force launch with HIP events
+    on the context's stream; the pair count is that launch's."""
+    from mysph_amd.sphgpu import Simulation
+    cfg = case["config"].copy()
+    cfg.precision = precision
+    ms, pairs = [], []
+    with Simulation(case, cfg) as sim:
+        sim.advance(warmup)
+        for _ in range(steps // every):
+            sim.advance(every)
+            ms.append(sim.timing()[1])
+            pairs.append(sim.neighbour_stats()["pairs"])
+    return float(np.mean(ms)), float(np.mean(pairs)), len(ms)
+
+
+def accuracy(case, nsteps, states=None):
     """Mixed against FP64 on the GPU after nsteps from the same state (the FP64
-    GPU path is checked against the oracle by the parity tests)."""
+    GPU path is checked against the oracle by the parity tests). `states`:
+    the end states of the timed runs when they are nsteps long."""
     from mysph_amd import config as C
     from mysph_amd.sphgpu import Simulation
     nf = case["n_real"]
-    out = {}
+    out = dict(states) if states else {}
     for name, prec in (("fp64", C.FP64), ("mixed", C.MIXED)):
+        if name in out:
+            continue
         cfg = case["config"].copy()
         cfg.precision = prec
         with Simulation(case, cfg) as sim:
@@ -211,7 +234,8 @@ def serving_leg(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    # the workload is 1000 steps with steps 11-1000 timed
+    ap.add_argument("--steps", type=int, default=990)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.0075)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -259,6 +283,7 @@ def main():
             continue
         r = run_gpu(case, prec, args.warmup, args.steps)
         r["pss"] = nf * args.steps / r["wall_s"]
+        r["force_ms"], r["pairs"], r["force_samples"] = force_samples(case, prec, args.warmup, args.steps)
         res[name] = r
         log(f"{name}: {r['pss']:.4g} particle-steps/s, {1e3 * r['wall_s'] / args.steps:.3f} ms/step, "
             f"force kernel {r['force_ms']:.3f} ms, pairs {r['pairs']}")
@@ -300,6 +325,7 @@ def main():
             "flop_per_pair": FLOP_PER_PAIR_3D,
             "pairs_per_launch": head["pairs"],
             "launch_ms": head["force_ms"],
+            "launch_samples": head["force_samples"],
             "hbm_gbs_measured": (traffic / (head["force_ms"] * 1e-3) / 1e9
                                  if traffic and head["force_ms"] > 0 else None),
             "hbm_frac_measured": (traffic / (head["force_ms"] * 1e-3) / PEAK_HBM
@@ -317,7 +343,9 @@ def main():
         d = np.max(np.abs(res["mixed"]["rho"] - res["fp64"]["rho"]) / res["fp64"]["rho"])
         out["mixed_rho_rel_err_vs_fp64"] = float(d)
     if "fp64" in res and "mixed" in res and args.accuracy_steps > 0:
-        out["accuracy"] = accuracy(case, args.accuracy_steps)
+        same = args.warmup + args.steps == args.accuracy_steps
+        out["accuracy"] = accuracy(case, args.accuracy_steps,
+                                   {k: res[k]["state"] for k in ("fp64", "mixed")} if same else None)
     if not args.no_serving:
         out["serving"] = serving_leg(args)
     if not args.no_cpu:
