@@ -79,12 +79,7 @@ def run_gpu(case, precision, warmup, steps):
     cfg = case["config"].copy()
     cfg.precision = precision
     sim = Simulation(case, cfg)
-    sim.advance(warmup)
-    try:
-        import torch
-        torch.cuda.synchronize()
-    except Exception:
-        pass
+    sim.advance(warmup)   # returns after the context's stream has drained
     t0 = time.perf_counter()
     sim.advance(steps)
     t1 = time.perf_counter()
@@ -114,6 +109,42 @@ def force_samples(case, precision, warmup, steps, every=10):
             ms.append(sim.timing()[1])
             pairs.append(sim.neighbour_stats()["pairs"])
     return float(np.mean(ms)), float(np.mean(pairs)), len(ms)
+
+
+def pcie_inclusive(case, reps=3):
+    """Per-step cost of the drop-in `single_step` at this size, where the
+    host state crosses PCIe on every call: upload every particle array,
+    evaluate the derivatives (search, EOS, force pass, fixed boundary), then
+    download what the drop-in writes back for this fluid case, called without
+    the solid-only Shear argument (dv/dt, de/dt, drho/dt, p, the two
+    divergences). Host arrays are allocated once, as the drop-in's are; its
+    Fortran packing loops are not included. FP64, the reference's arithmetic.
+    The first call (context set-up) is untimed."""
+    from mysph_amd import config as C
+    from mysph_amd.sphgpu import Simulation, _dp
+    names = ("dvdt", "dedt", "drhodt", "p", "divx", "divv")
+    with Simulation(case) as sim:
+        bufs = {k: np.zeros(sim.n * C.field_components(C.FIELD_NAMES[k], sim.dim)) for k in names}
+        t = np.zeros((reps, 3))
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            sim.upload(case)      # each call returns after its stream has drained
+            t1 = time.perf_counter()
+            sim.derivatives()
+            t2 = time.perf_counter()
+            for k in names:
+                sim._check(sim._L.sphgpu_download(sim._ctx, C.FIELD_NAMES[k], _dp(bufs[k])))
+            t3 = time.perf_counter()
+            if r:
+                t[r - 1] = (t1 - t0, t2 - t1, t3 - t2)
+        up = sum(a.nbytes for a in sim._keep.values() if a is not None)
+    ms = 1e3 * t.mean(axis=0)
+    step = float(ms.sum())
+    return {"value": case["n_real"] / (step * 1e-3), "unit": "particle-steps/s",
+            "ms_per_step": step, "upload_ms": float(ms[0]), "derivatives_ms": float(ms[1]),
+            "download_ms": float(ms[2]), "bytes_up": int(up),
+            "bytes_down": int(sum(b.nbytes for b in bufs.values())), "calls": reps,
+            "path": "drop-in single_step per call: upload + derivatives + downloads, f64"}
 
 
 def accuracy(case, nsteps, states=None):
@@ -244,6 +275,8 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed run of both precisions to report the mixed density error")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched serving leg")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="skip the per-call upload/download rate of the drop-in single_step")
     ap.add_argument("--serving-runs", type=int, default=256)
     ap.add_argument("--serving-steps", type=int, default=2000)
     ap.add_argument("--serving-slots", type=int, default=8_000_000,
@@ -346,6 +379,9 @@ def main():
         same = args.warmup + args.steps == args.accuracy_steps
         out["accuracy"] = accuracy(case, args.accuracy_steps,
                                    {k: res[k]["state"] for k in ("fp64", "mixed")} if same else None)
+    if not args.no_pcie:
+        out["pcie_inclusive"] = pcie_inclusive(case)
+        log(f"drop-in single_step with host buffers: {out['pcie_inclusive']['ms_per_step']:.2f} ms/step")
     if not args.no_serving:
         out["serving"] = serving_leg(args)
     if not args.no_cpu:

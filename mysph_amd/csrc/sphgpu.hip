@@ -1478,6 +1478,53 @@ struct Buf {
 
 }  // namespace
 
+// One run's particles in slot order with its parameters and grid plan: what
+// an upload copies to the device. Slots are ordered by our linear cell key (x
+// fastest): neighbouring lanes then hold x-neighbouring particles, so the k-th
+// neighbours of a wavefront's lanes sit in consecutive slots and the gathers
+// coalesce. A context keeps its HostRun between uploads, so a host that
+// uploads every step (the drop-in single_step) reuses these buffers.
+struct HostRun {
+    Params P{};
+    int64_t n = 0, n_real = 0, n_dummy = 0;
+    bool fast = false;
+    std::vector<double4> posh, vm;
+    std::vector<double> rho, e, c, p, shear;   // shear empty when the host passes none
+    std::vector<int4> meta;
+    std::vector<float> drho;   // mixed precision: rho - rho0
+    std::vector<uint32_t> sort_key, sort_start;   // counting-sort work space
+    std::vector<int64_t> order;                   // slot -> particle index
+};
+
+// Host staging for downloads: pinned (hipHostMalloc, grown on demand, kept
+// by the context) or pageable.
+struct HostStage {
+    bool pinned = false;
+    void* p = nullptr;
+    size_t bytes = 0;
+    std::vector<char> v;
+    void* get(size_t need) {
+        if (!pinned) {
+            if (v.size() < need) v.resize(need);
+            return v.data();
+        }
+        if (need > bytes) {
+            release();
+            if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
+                p = nullptr;
+                return nullptr;
+            }
+            bytes = need;
+        }
+        return p;
+    }
+    void release() {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
 struct sphgpu_ctx {
     sphgpu_config cfg{};
     Params P{};
@@ -1505,6 +1552,13 @@ struct sphgpu_ctx {
     hipGraphExec_t graph = nullptr;   // kGraphSteps fast-path steps, valid until the next upload
     std::string err;
     int* h_err = nullptr;   // pinned
+    // device buffers of the previous upload, reused by the next one when a
+    // buffer of the same size is asked for (freed at the end of that upload)
+    std::vector<void*> spare;
+    std::vector<size_t> spare_bytes;
+    HostRun up;              // the last upload in slot order; up.meta mirrors D.meta's
+    bool up_meta = false;    // type and index columns while it holds this state
+    HostStage dstage{true};  // pinned staging of downloads
 };
 
 static int fail(sphgpu_ctx* c, int code, const std::string& msg) {
@@ -1523,7 +1577,14 @@ template <typename T>
 static int dalloc(sphgpu_ctx* c, T** p, size_t count) {
     void* q = nullptr;
     size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-    HIPCHK(c, hipMalloc(&q, bytes));
+    for (size_t k = 0; k < c->spare.size(); ++k)
+        if (c->spare_bytes[k] == bytes) {
+            q = c->spare[k];
+            c->spare.erase(c->spare.begin() + k);
+            c->spare_bytes.erase(c->spare_bytes.begin() + k);
+            break;
+        }
+    if (!q) HIPCHK(c, hipMalloc(&q, bytes));
     HIPCHK(c, hipMemsetAsync(q, 0, bytes, c->stream));
     c->allocs.push_back(q);
     c->alloc_bytes.push_back(bytes);
@@ -1531,14 +1592,28 @@ static int dalloc(sphgpu_ctx* c, T** p, size_t count) {
     return 0;
 }
 
-static void free_all(sphgpu_ctx* c) {
-    for (void* q : c->allocs) hipFree(q);
+static void drop_spare(sphgpu_ctx* c) {
+    for (void* q : c->spare) hipFree(q);
+    c->spare.clear();
+    c->spare_bytes.clear();
+}
+
+// Every allocation becomes a spare for the allocations that follow (which
+// zero what they take, as fresh ones are zeroed).
+static void recycle_all(sphgpu_ctx* c) {
+    c->spare.insert(c->spare.end(), c->allocs.begin(), c->allocs.end());
+    c->spare_bytes.insert(c->spare_bytes.end(), c->alloc_bytes.begin(), c->alloc_bytes.end());
     c->allocs.clear();
     c->alloc_bytes.clear();
     c->D = Dev{};
     c->ncell_alloc = 0;
     c->nbr_alloc = 0;
     c->vnbr_alloc = 0;
+}
+
+static void free_all(sphgpu_ctx* c) {
+    recycle_all(c);
+    drop_spare(c);
 }
 
 static int validate_config(const sphgpu_config* cfg, std::string* why) {
@@ -2008,7 +2083,7 @@ static int check_err(sphgpu_ctx* c) {
 }
 
 static int alloc_state(sphgpu_ctx* c) {
-    free_all(c);
+    recycle_all(c);
     const int64_t N = c->capacity;
     const int dim = c->dim;
     const int64_t T = N * dim * dim;
@@ -2140,21 +2215,6 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     return 0;
 }
 
-// One run's particles in slot order with its parameters and grid plan: what
-// an upload copies to the device. Slots are ordered by our linear cell key (x
-// fastest): neighbouring lanes then hold x-neighbouring particles, so the k-th
-// neighbours of a wavefront's lanes sit in consecutive slots and the gathers
-// coalesce.
-struct HostRun {
-    Params P{};
-    int64_t n = 0, n_real = 0, n_dummy = 0;
-    bool fast = false;
-    std::vector<double4> posh, vm;
-    std::vector<double> rho, e, c, p, shear;
-    std::vector<int4> meta;
-    std::vector<float> drho;   // mixed precision: rho - rho0
-};
-
 static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy, const double* x,
                        const double* v, const double* mass, const double* rho, const double* p,
                        const double* e, const double* cs, const double* h, const double* shear,
@@ -2208,7 +2268,13 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
         for (int64_t i = 0; i < n_real; ++i)
             if (rho[i] > 0) h_next_max = std::max(h_next_max, 2.0 * std::pow(mass[i] / rho[i], 1.0 / dim));
     plan_grid_params(P, dim, hr.fast && use_verlet, n, x, h, tune, h_next_max);
-    std::vector<std::pair<uint64_t, int64_t>> keyed(n);
+    // slots by cell key, ascending particle index within a cell: a stable
+    // counting sort over the P.ncell (< 2^31) cells
+    std::vector<uint32_t>& key = hr.sort_key;
+    std::vector<uint32_t>& start = hr.sort_start;
+    std::vector<int64_t>& order = hr.order;
+    key.resize(n);
+    start.assign((size_t)P.ncell + 1, 0u);
     for (int64_t i = 0; i < n; ++i) {
         int64_t cc[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) {
@@ -2216,14 +2282,18 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
             t = std::min(std::max(t, 0.0), (double)(P.nc[d] - 1));
             cc[d] = (int64_t)t;
         }
-        keyed[i] = {(uint64_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2])), i};
+        key[i] = (uint32_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2]));
+        ++start[key[i] + 1];
     }
-    std::sort(keyed.begin(), keyed.end());
+    for (int64_t k = 0; k < P.ncell; ++k) start[k + 1] += start[k];
+    order.resize(n);
+    for (int64_t i = 0; i < n; ++i) order[start[key[i]]++] = i;
     hr.posh.resize(n); hr.vm.resize(n); hr.rho.resize(n); hr.e.resize(n); hr.c.resize(n);
     hr.p.resize(n); hr.meta.resize(n);
-    hr.shear.assign((size_t)n * dim * dim, 0.0);
+    if (shear) hr.shear.resize((size_t)n * dim * dim);
+    else hr.shear.clear();   // the device copy starts at zero
     for (int64_t s = 0; s < n; ++s) {
-        int64_t i = keyed[s].second;
+        int64_t i = order[s];
         hr.posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
         double vx = v ? v[i * dim] : 0, vy = v ? v[i * dim + 1] : 0, vz = (v && dim == 3) ? v[i * dim + 2] : 0;
         hr.vm[s] = make_double4(vx, vy, vz, mass[i]);
@@ -2257,7 +2327,8 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     if (!c) return SPHGPU_E_ARG;
     if (n_real + n_dummy > c->capacity)
         return fail(c, SPHGPU_E_CAPACITY, "more particles than the context capacity");
-    HostRun hr;
+    HostRun& hr = c->up;
+    c->up_meta = false;
     std::string why;
     int rc = prepare_run(c->cfg, n_real, n_dummy, x, v, mass, rho, p, e, cs, h, shear, type, state,
                          boundary, c->use_verlet, c->force_general, env_tuning(), hr, why);
@@ -2272,11 +2343,8 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     if (rc) return rc;
     rc = alloc_cells(c);
     if (rc) return rc;
-    std::vector<double4> disp(n), zero4(n);
-    for (int64_t s = 0; s < n; ++s) {
-        disp[s] = make_double4(0, 0, 0, 0);
-        zero4[s] = make_double4(0, 0, 0, 0);
-    }
+    // every state buffer starts at zero (alloc_state), so the displacement
+    // and the derivatives need no copy
     const std::vector<double4>& posh = hr.posh;
     const std::vector<double4>& vm = hr.vm;
     const std::vector<double>& vrho = hr.rho;
@@ -2288,9 +2356,6 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     Dev& D = c->D;
     HIPCHK(c, hipMemcpyAsync(D.posh, posh.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.vm, vm.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
-    if (!c->P.mixed)   // FP32 dispf starts at zero from its allocation
-        HIPCHK(c, hipMemcpyAsync(D.disp, disp.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(D.dv, zero4.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.rho, vrho.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.e, ve.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     std::vector<float> vef;
@@ -2301,7 +2366,9 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     HIPCHK(c, hipMemcpyAsync(D.c, vc.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.p, vp.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.meta, meta.data(), n * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(D.shear, vshear.data(), vshear.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!vshear.empty())
+        HIPCHK(c, hipMemcpyAsync(D.shear, vshear.data(), vshear.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
     if (c->P.mixed) {
         HIPCHK(c, hipMemcpyAsync(D.drho, hr.drho.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2314,6 +2381,9 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     drop_graph(c);
     rc = dim == 2 ? size_neighbour_list<2>(c) : size_neighbour_list<3>(c);
     if (rc) { c->has_state = false; return rc; }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_spare(c);
+    c->up_meta = true;
     // the grid partials of the uploaded positions seed the first step
     const int nb = (int)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
@@ -2387,13 +2457,36 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
         }                                                                    \
     } while (0)
 
+// One field of a run in particle order. `meta_host`: a host copy of the run's
+// type and index columns (D.meta .x and .w), or null to read them from the
+// device; `stage`: host staging for the copy, or null for a pageable buffer.
 static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sphgpu_config& cfg,
-                         hipStream_t stream, int32_t field, double* dst, std::string& msg) {
-    std::vector<int4> meta(n);
-    VCHK(hipMemcpyAsync(meta.data(), D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, stream));
+                         hipStream_t stream, int32_t field, double* dst, std::string& msg,
+                         const int4* meta_host = nullptr, HostStage* stage = nullptr) {
+    std::vector<int4> meta_dev;
+    const int4* meta = meta_host;
+    if (!meta) {
+        meta_dev.resize(n);
+        VCHK(hipMemcpyAsync(meta_dev.data(), D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, stream));
+        meta = meta_dev.data();
+    }
+    HostStage local;
+    HostStage& st = stage ? *stage : local;
+    bool staged = true;
+    auto buf = [&](size_t bytes) -> void* {
+        void* q = st.get(bytes);
+        staged = q != nullptr;
+        return q;
+    };
+#define STAGE(T, name, count)                                  \
+    T* name = static_cast<T*>(buf((count) * sizeof(T)));       \
+    if (!staged) {                                              \
+        msg = "host staging allocation failed";                 \
+        return SPHGPU_E_HIP;                                    \
+    }
     auto get4 = [&](double4* src, int which) -> int {  // which: 0 xyz, 1 w
-        std::vector<double4> h(n);
-        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(double4), hipMemcpyDeviceToHost, stream));
+        STAGE(double4, h, n);
+        VCHK(hipMemcpyAsync(h, src, n * sizeof(double4), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) {
             int64_t i = meta[s].w;
@@ -2408,8 +2501,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         return 0;
     };
     auto get1 = [&](double* src) -> int {
-        std::vector<double> h(n);
-        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        STAGE(double, h, n);
+        VCHK(hipMemcpyAsync(h, src, n * sizeof(double), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = h[s];
         return 0;
@@ -2420,16 +2513,16 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
             std::memset(dst, 0, sizeof(double) * n * nn);
             return 0;
         }
-        std::vector<double> h(n * nn);
-        VCHK(hipMemcpyAsync(h.data(), src, n * nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+        STAGE(double, h, n * nn);
+        VCHK(hipMemcpyAsync(h, src, n * nn * sizeof(double), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s)
             for (int q = 0; q < nn; ++q) dst[meta[s].w * nn + q] = h[s * nn + q];
         return 0;
     };
     auto getf4 = [&](float4* src, int which) -> int {  // which: 0 xyz, 1 w
-        std::vector<float4> h(n);
-        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
+        STAGE(float4, h, n);
+        VCHK(hipMemcpyAsync(h, src, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) {
             int64_t i = meta[s].w;
@@ -2444,8 +2537,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         return 0;
     };
     auto getf1 = [&](float* src, bool add_rho0) -> int {
-        std::vector<float> h(n);
-        VCHK(hipMemcpyAsync(h.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        STAGE(float, h, n);
+        VCHK(hipMemcpyAsync(h, src, n * sizeof(float), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) {
             double r0 = 0.0;
@@ -2467,8 +2560,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         case SPHGPU_F_E: return getf1(D.ef, false);
         case SPHGPU_F_DIVX:
         case SPHGPU_F_DIVV: {
-            std::vector<float2> h(n);
-            VCHK(hipMemcpyAsync(h.data(), D.divf, n * sizeof(float2), hipMemcpyDeviceToHost, stream));
+            STAGE(float2, h, n);
+            VCHK(hipMemcpyAsync(h, D.divf, n * sizeof(float2), hipMemcpyDeviceToHost, stream));
             VCHK(hipStreamSynchronize(stream));
             for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = field == SPHGPU_F_DIVX ? h[s].x : h[s].y;
             return 0;
@@ -2478,8 +2571,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         }
     }
     auto get2 = [&](int which) -> int {
-        std::vector<double2> h(n);
-        VCHK(hipMemcpyAsync(h.data(), D.div, n * sizeof(double2), hipMemcpyDeviceToHost, stream));
+        STAGE(double2, h, n);
+        VCHK(hipMemcpyAsync(h, D.div, n * sizeof(double2), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
         for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = which ? h[s].y : h[s].x;
         return 0;
@@ -2505,6 +2598,7 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     case SPHGPU_F_DIVV: return get2(1);
     default: break;
     }
+#undef STAGE
     msg = std::string("field not available: ") + kFieldNames[field];
     return SPHGPU_E_ARG;
 }
@@ -2514,7 +2608,9 @@ extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     if (!dst || field < 0 || field >= SPHGPU_F_NFIELDS) return fail(c, SPHGPU_E_ARG, "bad field");
     std::string msg;
-    int rc = download_view(c->D, c->n, c->dim, c->P.mixed != 0, c->cfg, c->stream, field, dst, msg);
+    const int4* meta = c->up_meta && (int64_t)c->up.meta.size() == c->n ? c->up.meta.data() : nullptr;
+    int rc = download_view(c->D, c->n, c->dim, c->P.mixed != 0, c->cfg, c->stream, field, dst, msg, meta,
+                           &c->dstage);
     return rc ? fail(c, rc, msg) : 0;
 }
 
@@ -2620,6 +2716,7 @@ extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_graph(c);
     free_all(c);
+    c->dstage.release();
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->h_err) hipHostFree(c->h_err);

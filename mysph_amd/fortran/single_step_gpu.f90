@@ -74,8 +74,9 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
     real(8), intent(inout), optional :: dSdt(:, :, :)
 
     type(sphgpu_config) :: cfg
-    integer :: N, dim, i, a, b, k
+    integer :: N, dim, i, a, b
     integer(c_int64_t) :: st(4)
+    type(c_ptr) :: pshear
 
     dim = Field%Dim
     if (Config%dummy_parti_w) call gen_dummy_particle(ntotal, ndummy, Particles)
@@ -122,17 +123,20 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
         stype(i)  = Particles(i)%Type
         sstate(i) = Particles(i)%State
         sbnd(i)   = Particles(i)%Boundary
-        do a = 1, dim
-            do b = 1, dim
-                k = (i-1)*dim*dim + (a-1)*dim + b
-                sshear(k) = 0
-                if (present(Shear)) sshear(k) = Shear(a, b, i)
+        if (present(Shear)) then
+            do a = 1, dim
+                do b = 1, dim
+                    sshear((i-1)*dim*dim + (a-1)*dim + b) = Shear(a, b, i)
+                end do
             end do
-        end do
+        end if
     end do
+    !! without Shear the library's deviatoric stress starts at zero
+    pshear = c_null_ptr
+    if (present(Shear)) pshear = c_loc(sshear)
     call sphgpu_check(sphgpu_upload(gpu_ctx, int(ntotal, c_int64_t), int(ndummy, c_int64_t), &
                       c_loc(sx), c_loc(sv), c_loc(smass), c_loc(srho), c_loc(sp), c_loc(se), &
-                      c_loc(sc), c_loc(sh), c_loc(sshear), c_loc(stype), c_loc(sstate), c_loc(sbnd)), &
+                      c_loc(sc), c_loc(sh), pshear, c_loc(stype), c_loc(sstate), c_loc(sbnd)), &
                       gpu_ctx, "sphgpu_upload")
     call sphgpu_check(sphgpu_derivatives(gpu_ctx), gpu_ctx, "sphgpu_derivatives")
 
@@ -180,15 +184,17 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
         end do
     end if
     !! Stress = Shear - p I for solids only (internalForce.f90:175-180)
-    call sphgpu_check(sphgpu_download(gpu_ctx, SPHGPU_F_STRESS, c_loc(sout)), gpu_ctx, "download stress")
-    do i = 1, N
-        if (abs(Particles(i)%Type) <= 100 .or. .not. present(Shear)) cycle
-        do a = 1, dim
-            do b = 1, dim
-                Particles(i)%Stress(a, b) = sout((i-1)*dim*dim + (a-1)*dim + b)
+    if (present(Shear)) then
+        call sphgpu_check(sphgpu_download(gpu_ctx, SPHGPU_F_STRESS, c_loc(sout)), gpu_ctx, "download stress")
+        do i = 1, N
+            if (abs(Particles(i)%Type) <= 100) cycle
+            do a = 1, dim
+                do b = 1, dim
+                    Particles(i)%Stress(a, b) = sout((i-1)*dim*dim + (a-1)*dim + b)
+                end do
             end do
         end do
-    end do
+    end if
     call sphgpu_check(sphgpu_download(gpu_ctx, SPHGPU_F_DIVX, c_loc(sout)), gpu_ctx, "download divx")
     do i = 1, ntotal
         Particles(i)%divergencePosition = sout(i)
