@@ -1494,6 +1494,7 @@ struct HostRun {
     std::vector<float> drho;   // mixed precision: rho - rho0
     std::vector<uint32_t> sort_key, sort_start;   // counting-sort work space
     std::vector<int64_t> order;                   // slot -> particle index
+    std::vector<int32_t> type_idx;                // type by particle index
 };
 
 // Host staging for downloads: pinned (hipHostMalloc, grown on demand, kept
@@ -1556,9 +1557,10 @@ struct sphgpu_ctx {
     // buffer of the same size is asked for (freed at the end of that upload)
     std::vector<void*> spare;
     std::vector<size_t> spare_bytes;
-    HostRun up;              // the last upload in slot order; up.meta mirrors D.meta's
-    bool up_meta = false;    // type and index columns while it holds this state
+    HostRun up;              // the last upload in slot order
+    bool up_meta = false;    // up.type_idx holds the uploaded particles' types
     HostStage dstage{true};  // pinned staging of downloads
+    void* dorder = nullptr;  // device scratch: a downloaded field in particle order
 };
 
 static int fail(sphgpu_ctx* c, int code, const std::string& msg) {
@@ -2288,6 +2290,7 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     for (int64_t k = 0; k < P.ncell; ++k) start[k + 1] += start[k];
     order.resize(n);
     for (int64_t i = 0; i < n; ++i) order[start[key[i]]++] = i;
+    hr.type_idx.assign(type, type + n);
     hr.posh.resize(n); hr.vm.resize(n); hr.rho.resize(n); hr.e.resize(n); hr.c.resize(n);
     hr.p.resize(n); hr.meta.resize(n);
     if (shear) hr.shear.resize((size_t)n * dim * dim);
@@ -2457,19 +2460,55 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
         }                                                                    \
     } while (0)
 
-// One field of a run in particle order. `meta_host`: a host copy of the run's
-// type and index columns (D.meta .x and .w), or null to read them from the
-// device; `stage`: host staging for the copy, or null for a pageable buffer.
+// A slot-ordered field in particle order: dst[meta[s].w] = src[s] for
+// elements of `words` 32-bit words (a run's meta.w is a permutation of 0..n-1).
+__global__ __launch_bounds__(kBlock) void k_particle_order(const uint32_t* __restrict__ src,
+                                                           const int4* __restrict__ meta, int64_t n,
+                                                           int words, uint32_t* __restrict__ dst) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = meta[s].w;
+    if (i < 0 || i >= n) return;
+    for (int q = 0; q < words; ++q) dst[i * words + q] = src[s * words + q];
+}
+
+// One field of a run in particle order. `order`: device scratch of n
+// elements of the widest field (n * dim * dim doubles); with it the field is
+// put in particle order on the device and `type_by_index` gives each
+// particle's type (for the mixed density), else the host permutes with the
+// run's meta read from the device. `stage`: host staging for the copy, or
+// null for a pageable buffer.
 static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sphgpu_config& cfg,
                          hipStream_t stream, int32_t field, double* dst, std::string& msg,
-                         const int4* meta_host = nullptr, HostStage* stage = nullptr) {
+                         void* order = nullptr, const int32_t* type_by_index = nullptr,
+                         HostStage* stage = nullptr) {
+    const bool ordered = order != nullptr && type_by_index != nullptr;
     std::vector<int4> meta_dev;
-    const int4* meta = meta_host;
-    if (!meta) {
+    const int4* meta = nullptr;
+    if (!ordered) {
         meta_dev.resize(n);
         VCHK(hipMemcpyAsync(meta_dev.data(), D.meta, n * sizeof(int4), hipMemcpyDeviceToHost, stream));
         meta = meta_dev.data();
     }
+    auto idx = [&](int64_t s) -> int64_t { return ordered ? s : (int64_t)meta[s].w; };
+    auto typ = [&](int64_t s) -> int { return ordered ? type_by_index[s] : meta[s].x; };
+    // copy n elements of `elem` bytes into h, in particle order when ordered
+    auto fetch = [&](void* h, const void* src, size_t elem) -> int {
+        if (!src) {
+            msg = std::string("field not held by this run: ") + kFieldNames[field];
+            return SPHGPU_E_ARG;
+        }
+        if (ordered) {
+            hipLaunchKernelGGL(k_particle_order, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               stream, static_cast<const uint32_t*>(src), D.meta, n, (int)(elem / 4),
+                               static_cast<uint32_t*>(order));
+            VCHK(hipGetLastError());
+            src = order;
+        }
+        VCHK(hipMemcpyAsync(h, src, n * elem, hipMemcpyDeviceToHost, stream));
+        VCHK(hipStreamSynchronize(stream));
+        return 0;
+    };
     HostStage local;
     HostStage& st = stage ? *stage : local;
     bool staged = true;
@@ -2486,10 +2525,9 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     }
     auto get4 = [&](double4* src, int which) -> int {  // which: 0 xyz, 1 w
         STAGE(double4, h, n);
-        VCHK(hipMemcpyAsync(h, src, n * sizeof(double4), hipMemcpyDeviceToHost, stream));
-        VCHK(hipStreamSynchronize(stream));
+        if (int rc_ = fetch(h, src, sizeof(double4))) return rc_;
         for (int64_t s = 0; s < n; ++s) {
-            int64_t i = meta[s].w;
+            int64_t i = idx(s);
             if (which == 0) {
                 dst[i * dim] = h[s].x;
                 dst[i * dim + 1] = h[s].y;
@@ -2502,9 +2540,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     };
     auto get1 = [&](double* src) -> int {
         STAGE(double, h, n);
-        VCHK(hipMemcpyAsync(h, src, n * sizeof(double), hipMemcpyDeviceToHost, stream));
-        VCHK(hipStreamSynchronize(stream));
-        for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = h[s];
+        if (int rc_ = fetch(h, src, sizeof(double))) return rc_;
+        for (int64_t s = 0; s < n; ++s) dst[idx(s)] = h[s];
         return 0;
     };
     auto getT = [&](double* src) -> int {
@@ -2514,18 +2551,16 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
             return 0;
         }
         STAGE(double, h, n * nn);
-        VCHK(hipMemcpyAsync(h, src, n * nn * sizeof(double), hipMemcpyDeviceToHost, stream));
-        VCHK(hipStreamSynchronize(stream));
+        if (int rc_ = fetch(h, src, nn * sizeof(double))) return rc_;
         for (int64_t s = 0; s < n; ++s)
-            for (int q = 0; q < nn; ++q) dst[meta[s].w * nn + q] = h[s * nn + q];
+            for (int q = 0; q < nn; ++q) dst[idx(s) * nn + q] = h[s * nn + q];
         return 0;
     };
     auto getf4 = [&](float4* src, int which) -> int {  // which: 0 xyz, 1 w
         STAGE(float4, h, n);
-        VCHK(hipMemcpyAsync(h, src, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
-        VCHK(hipStreamSynchronize(stream));
+        if (int rc_ = fetch(h, src, sizeof(float4))) return rc_;
         for (int64_t s = 0; s < n; ++s) {
-            int64_t i = meta[s].w;
+            int64_t i = idx(s);
             if (which == 0) {
                 dst[i * dim] = h[s].x;
                 dst[i * dim + 1] = h[s].y;
@@ -2538,16 +2573,15 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     };
     auto getf1 = [&](float* src, bool add_rho0) -> int {
         STAGE(float, h, n);
-        VCHK(hipMemcpyAsync(h, src, n * sizeof(float), hipMemcpyDeviceToHost, stream));
-        VCHK(hipStreamSynchronize(stream));
+        if (int rc_ = fetch(h, src, sizeof(float))) return rc_;
         for (int64_t s = 0; s < n; ++s) {
             double r0 = 0.0;
             if (add_rho0) {
-                int a = std::abs(meta[s].x);
+                int a = std::abs(typ(s));
                 for (int k = 0; k < cfg.n_materials; ++k)
                     if (cfg.mat[k].type == a) r0 = cfg.mat[k].rho0;
             }
-            dst[meta[s].w] = r0 + (double)h[s];
+            dst[idx(s)] = r0 + (double)h[s];
         }
         return 0;
     };
@@ -2561,9 +2595,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         case SPHGPU_F_DIVX:
         case SPHGPU_F_DIVV: {
             STAGE(float2, h, n);
-            VCHK(hipMemcpyAsync(h, D.divf, n * sizeof(float2), hipMemcpyDeviceToHost, stream));
-            VCHK(hipStreamSynchronize(stream));
-            for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = field == SPHGPU_F_DIVX ? h[s].x : h[s].y;
+            if (int rc_ = fetch(h, D.divf, sizeof(float2))) return rc_;
+            for (int64_t s = 0; s < n; ++s) dst[idx(s)] = field == SPHGPU_F_DIVX ? h[s].x : h[s].y;
             return 0;
         }
         case SPHGPU_F_DEDT: return getf1(D.def, false);
@@ -2572,9 +2605,8 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     }
     auto get2 = [&](int which) -> int {
         STAGE(double2, h, n);
-        VCHK(hipMemcpyAsync(h, D.div, n * sizeof(double2), hipMemcpyDeviceToHost, stream));
-        VCHK(hipStreamSynchronize(stream));
-        for (int64_t s = 0; s < n; ++s) dst[meta[s].w] = which ? h[s].y : h[s].x;
+        if (int rc_ = fetch(h, D.div, sizeof(double2))) return rc_;
+        for (int64_t s = 0; s < n; ++s) dst[idx(s)] = which ? h[s].y : h[s].x;
         return 0;
     };
     switch (field) {
@@ -2608,9 +2640,12 @@ extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
     if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
     if (!dst || field < 0 || field >= SPHGPU_F_NFIELDS) return fail(c, SPHGPU_E_ARG, "bad field");
     std::string msg;
-    const int4* meta = c->up_meta && (int64_t)c->up.meta.size() == c->n ? c->up.meta.data() : nullptr;
-    int rc = download_view(c->D, c->n, c->dim, c->P.mixed != 0, c->cfg, c->stream, field, dst, msg, meta,
-                           &c->dstage);
+    // device scratch for the reorder, the size of the widest field at capacity
+    const size_t obytes = (size_t)c->capacity * c->dim * c->dim * sizeof(double);
+    if (c->up_meta && !c->dorder) HIPCHK(c, hipMalloc(&c->dorder, obytes));
+    const bool typed = c->up_meta && (int64_t)c->up.type_idx.size() == c->n;
+    int rc = download_view(c->D, c->n, c->dim, c->P.mixed != 0, c->cfg, c->stream, field, dst, msg,
+                           typed ? c->dorder : nullptr, typed ? c->up.type_idx.data() : nullptr, &c->dstage);
     return rc ? fail(c, rc, msg) : 0;
 }
 
@@ -2717,6 +2752,7 @@ extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     drop_graph(c);
     free_all(c);
     c->dstage.release();
+    if (c->dorder) hipFree(c->dorder);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->h_err) hipHostFree(c->h_err);
