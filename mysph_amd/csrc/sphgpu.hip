@@ -28,6 +28,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -2217,6 +2218,26 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     return 0;
 }
 
+// f(lo, hi) over [0, n) split across up to 16 host threads (one thread for
+// small n); each call owns its range, so the result does not depend on the
+// split.
+template <typename F>
+static void host_parallel(int64_t n, F&& f) {
+    const int64_t t = std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < (1 << 16) || t == 1) {
+        f((int64_t)0, n);
+        return;
+    }
+    const int64_t chunk = (n + t - 1) / t;
+    std::vector<std::thread> pool;
+    for (int64_t k = 1; k < t; ++k) {
+        const int64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi) pool.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    f((int64_t)0, std::min(n, chunk));
+    for (auto& th : pool) th.join();
+}
+
 static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy, const double* x,
                        const double* v, const double* mass, const double* rho, const double* p,
                        const double* e, const double* cs, const double* h, const double* shear,
@@ -2276,17 +2297,19 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     std::vector<uint32_t>& start = hr.sort_start;
     std::vector<int64_t>& order = hr.order;
     key.resize(n);
-    start.assign((size_t)P.ncell + 1, 0u);
-    for (int64_t i = 0; i < n; ++i) {
-        int64_t cc[3] = {0, 0, 0};
-        for (int d = 0; d < dim; ++d) {
-            double t = std::floor((x[i * dim + d] - P.org[d]) * P.inv_cs);
-            t = std::min(std::max(t, 0.0), (double)(P.nc[d] - 1));
-            cc[d] = (int64_t)t;
+    host_parallel(n, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            int64_t cc[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) {
+                double t = std::floor((x[i * dim + d] - P.org[d]) * P.inv_cs);
+                t = std::min(std::max(t, 0.0), (double)(P.nc[d] - 1));
+                cc[d] = (int64_t)t;
+            }
+            key[i] = (uint32_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2]));
         }
-        key[i] = (uint32_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2]));
-        ++start[key[i] + 1];
-    }
+    });
+    start.assign((size_t)P.ncell + 1, 0u);
+    for (int64_t i = 0; i < n; ++i) ++start[key[i] + 1];
     for (int64_t k = 0; k < P.ncell; ++k) start[k + 1] += start[k];
     order.resize(n);
     for (int64_t i = 0; i < n; ++i) order[start[key[i]]++] = i;
@@ -2295,19 +2318,21 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     hr.p.resize(n); hr.meta.resize(n);
     if (shear) hr.shear.resize((size_t)n * dim * dim);
     else hr.shear.clear();   // the device copy starts at zero
-    for (int64_t s = 0; s < n; ++s) {
-        int64_t i = order[s];
-        hr.posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
-        double vx = v ? v[i * dim] : 0, vy = v ? v[i * dim + 1] : 0, vz = (v && dim == 3) ? v[i * dim + 2] : 0;
-        hr.vm[s] = make_double4(vx, vy, vz, mass[i]);
-        hr.rho[s] = rho[i];
-        hr.e[s] = e ? e[i] : 0.0;
-        hr.c[s] = cs ? cs[i] : 0.0;
-        hr.p[s] = p ? p[i] : 0.0;
-        hr.meta[s] = make_int4(type[i], state ? state[i] : 0, boundary ? boundary[i] : 0, (int)i);
-        if (shear)
-            for (int q = 0; q < dim * dim; ++q) hr.shear[s * dim * dim + q] = shear[i * dim * dim + q];
-    }
+    host_parallel(n, [&](int64_t lo, int64_t hi) {
+        for (int64_t s = lo; s < hi; ++s) {
+            int64_t i = order[s];
+            hr.posh[s] = make_double4(x[i * dim], x[i * dim + 1], dim == 3 ? x[i * dim + 2] : 0.0, h[i]);
+            double vx = v ? v[i * dim] : 0, vy = v ? v[i * dim + 1] : 0, vz = (v && dim == 3) ? v[i * dim + 2] : 0;
+            hr.vm[s] = make_double4(vx, vy, vz, mass[i]);
+            hr.rho[s] = rho[i];
+            hr.e[s] = e ? e[i] : 0.0;
+            hr.c[s] = cs ? cs[i] : 0.0;
+            hr.p[s] = p ? p[i] : 0.0;
+            hr.meta[s] = make_int4(type[i], state ? state[i] : 0, boundary ? boundary[i] : 0, (int)i);
+            if (shear)
+                for (int q = 0; q < dim * dim; ++q) hr.shear[s * dim * dim + q] = shear[i * dim * dim + q];
+        }
+    });
     if (P.mixed) {
         hr.drho.resize(n);
         for (int64_t s = 0; s < n; ++s) {
