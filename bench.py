@@ -9,8 +9,10 @@ and the integrator update, with every particle resident in HBM.
 
 Prints ONE JSON line. ``value`` is the mixed-precision particle-steps/s
 (fluid particles x steps / s); the FP64 figure, their ratio, the density
-difference between them, the force kernel's roofline and a CPU baseline
-(the oracle restatement on the host cores, on a bounded sample) ride along.
+difference between them, the force kernel's roofline, the per-call rate of
+the drop-in single_step with host buffers (PCIe and host work included), the
+batched serving leg and a CPU baseline (the oracle restatement on the host
+cores, on a bounded sample) ride along.
 """
 from __future__ import annotations
 
