@@ -2232,7 +2232,12 @@ static void host_parallel(int64_t n, F&& f) {
     std::vector<std::thread> pool;
     for (int64_t k = 1; k < t; ++k) {
         const int64_t lo = k * chunk, hi = std::min(n, lo + chunk);
-        if (lo < hi) pool.emplace_back([&f, lo, hi] { f(lo, hi); });
+        if (lo >= hi) continue;
+        try {
+            pool.emplace_back([&f, lo, hi] { f(lo, hi); });
+        } catch (...) {   // no thread to be had: this range runs here
+            f(lo, hi);
+        }
     }
     f((int64_t)0, std::min(n, chunk));
     for (auto& th : pool) th.join();
