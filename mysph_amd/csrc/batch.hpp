@@ -86,6 +86,7 @@ struct BatchRun {
     int64_t c0 = 0, ncells = 0;
     int64_t step = 1;        // number of the next step
     int64_t target = 0;      // last step of the run
+    std::vector<int32_t> type_idx;   // type by particle index (downloads)
 };
 
 constexpr int kBatchVerletCap2D = 48;
@@ -134,6 +135,9 @@ struct sphgpu_batch {
     Params* h_runs = nullptr;                // pinned staging of the run table
     int* h_err = nullptr;                    // pinned copy of the error words
     RangeAlloc slots, cells;
+    void* dorder = nullptr;      // device scratch: a downloaded field in particle order
+    size_t dorder_bytes = 0;
+    HostStage dstage{true};      // pinned staging of downloads
     std::vector<BatchRun> runs;
     std::vector<int2> blk;
     std::vector<int> rows;
@@ -174,6 +178,8 @@ static void batch_free(sphgpu_batch* b) {
     b->allocs.clear();
     if (b->h_runs) hipHostFree(b->h_runs);
     if (b->h_err) hipHostFree(b->h_err);
+    if (b->dorder) hipFree(b->dorder);
+    b->dstage.release();
     for (auto& e : b->ev)
         if (e) hipEventDestroy(e);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -330,6 +336,7 @@ extern "C" int sphgpu_batch_add(sphgpu_batch* b, const sphgpu_config* cfg, int64
     r.cfg = *cfg;
     r.n = hr.n;
     r.n_real = hr.n_real;
+    r.type_idx = std::move(hr.type_idx);
     r.s0 = s0;
     r.nslot = nslot;
     r.c0 = c0;
@@ -584,8 +591,24 @@ extern "C" int sphgpu_batch_download(sphgpu_batch* b, int32_t run, int32_t field
     off(V.dv); off(V.de); off(V.div); off(V.eta); off(V.drho); off(V.dvf); off(V.dispf);
     off(V.ef); off(V.divf);
     off(V.def);
+    // device scratch for the reorder: the widest field of this run (downloads
+    // synchronise, so nothing in flight uses the old buffer)
+    const size_t need = (size_t)r->n * b->dim * b->dim * sizeof(double);
+    if (need > b->dorder_bytes) {
+        if (b->dorder) hipFree(b->dorder);
+        b->dorder = nullptr;
+        b->dorder_bytes = 0;
+        hipError_t e = hipMalloc(&b->dorder, need);
+        if (e != hipSuccess) {
+            b->dorder = nullptr;
+            return bfail(b, SPHGPU_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        b->dorder_bytes = need;
+    }
+    const bool typed = (int64_t)r->type_idx.size() == r->n;
     std::string msg;
-    int rc = download_view(V, r->n, b->dim, b->precision == SPHGPU_MIXED, r->cfg, b->stream, field, dst, msg);
+    int rc = download_view(V, r->n, b->dim, b->precision == SPHGPU_MIXED, r->cfg, b->stream, field, dst, msg,
+                           typed ? b->dorder : nullptr, typed ? r->type_idx.data() : nullptr, &b->dstage);
     return rc ? bfail(b, rc, msg) : 0;
 }
 
