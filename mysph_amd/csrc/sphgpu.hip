@@ -2313,11 +2313,24 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
             key[i] = (uint32_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2]));
         }
     });
+    // each thread owns a range of cells and scans every particle in index
+    // order: the counts and placements of its cells are its own, and a cell
+    // keeps ascending indices
     start.assign((size_t)P.ncell + 1, 0u);
-    for (int64_t i = 0; i < n; ++i) ++start[key[i] + 1];
+    host_parallel(P.ncell, [&](int64_t klo, int64_t khi) {
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t k = key[i];
+            if (k >= klo && k < khi) ++start[k + 1];
+        }
+    });
     for (int64_t k = 0; k < P.ncell; ++k) start[k + 1] += start[k];
     order.resize(n);
-    for (int64_t i = 0; i < n; ++i) order[start[key[i]]++] = i;
+    host_parallel(P.ncell, [&](int64_t klo, int64_t khi) {
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t k = key[i];
+            if (k >= klo && k < khi) order[start[k]++] = i;
+        }
+    });
     hr.type_idx.assign(type, type + n);
     hr.posh.resize(n); hr.vm.resize(n); hr.rho.resize(n); hr.e.resize(n); hr.c.resize(n);
     hr.p.resize(n); hr.meta.resize(n);
