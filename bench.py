@@ -149,34 +149,115 @@ def pcie_inclusive(case, reps=3):
             "path": "drop-in single_step per call: upload + derivatives + downloads, f64"}
 
 
-def accuracy(case, nsteps, states=None):
+def accuracy(case, nsteps, states=None, warm=10):
     """Mixed against FP64 on the GPU after nsteps from the same state (the FP64
     GPU path is checked against the oracle by the parity tests). `states`:
-    the end states of the timed runs when they are nsteps long."""
+    the end states of the timed runs when they are nsteps long. A run made
+    here is timed as well (steps warm+1 .. nsteps, after `warm` untimed
+    steps), which gives the steady per-step figures of the whole workload
+    whatever window the timed runs used."""
     from mysph_amd import config as C
     from mysph_amd.sphgpu import Simulation
     nf = case["n_real"]
     out = dict(states) if states else {}
+    steady = {}
     for name, prec in (("fp64", C.FP64), ("mixed", C.MIXED)):
         if name in out:
             continue
         cfg = case["config"].copy()
         cfg.precision = prec
         with Simulation(case, cfg) as sim:
-            sim.advance(nsteps)
+            sim.advance(warm)
+            t0 = time.perf_counter()
+            sim.advance(nsteps - warm)
+            t1 = time.perf_counter()
+            steady[name] = 1e3 * (t1 - t0) / (nsteps - warm)
             out[name] = {k: sim.get(k)[:nf] for k in ("rho", "x", "v")}
     rho_err = np.max(np.abs(out["mixed"]["rho"] - out["fp64"]["rho"]) / out["fp64"]["rho"])
     x_err = np.max(np.abs(out["mixed"]["x"] - out["fp64"]["x"])) / np.max(np.abs(out["fp64"]["x"]))
     v_err = np.max(np.abs(out["mixed"]["v"] - out["fp64"]["v"])) / np.max(np.abs(out["fp64"]["v"]))
     return {"steps": nsteps, "mixed_vs_fp64_rho_max_rel": float(rho_err),
-            "mixed_vs_fp64_x_rel": float(x_err), "mixed_vs_fp64_v_rel": float(v_err)}
+            "mixed_vs_fp64_x_rel": float(x_err), "mixed_vs_fp64_v_rel": float(v_err)}, steady
+
+
+def host_cpu():
+    """The host CPU as lscpu reports it, and the cores this job may run on."""
+    info = {}
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = dict(line.split(":", 1) for line in txt.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        info["logical_cpus"] = int(kv.get("CPU(s)", "0") or 0)
+        cps = int(kv.get("Core(s) per socket", "0") or 0)
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        info["physical_cores"] = cps * sockets or None
+    except Exception as e:  # lscpu missing: the counts below still stand
+        info["lscpu_error"] = repr(e)
+    try:
+        info["cpus_allowed"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["cpus_allowed"] = os.cpu_count() or 1
+    return info
+
+
+def baseline_threads(info):
+    """Every core the job may use: the CPUs in its affinity mask, or fewer
+    when the environment caps OpenMP (a shared GPU host allots a share of its
+    cores to each GPU job)."""
+    n = info.get("cpus_allowed") or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return n
+
+
+def reference_calibration(threads, nsteps=100):
+    """The compiled reference (its own modules, built from source into
+    oracle/_ref/ref_driver) and the oracle port timed on the reference's own
+    2D dam break (15,376 fluid + 5,734 wall particles), same steps, same
+    threads: how the port's speed compares with the reference's."""
+    import subprocess
+    import tempfile
+    import oracle
+    from mysph_amd import cases
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    out = {"case": "reference 2D dam break (data/damBreak), 15376 fluid + 5734 walls",
+           "steps": nsteps, "threads": threads}
+    case = cases.reference_dam_break()
+    oracle.set_threads(threads)
+    sim = oracle.OracleSim(case)
+    t0 = time.perf_counter()
+    sim.run(nsteps)
+    out["port_s"] = time.perf_counter() - t0
+    out["port_value"] = case["n_real"] * nsteps / out["port_s"]
+    if not os.path.exists(exe):
+        out["reference_error"] = "oracle/_ref/ref_driver is not built"
+        return out
+    env = dict(os.environ, REF_THREADS=str(threads), OMP_NUM_THREADS=str(threads))
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "dam_break", "run", str(nsteps), os.path.join(d, "o.bin")],
+                           capture_output=True, text=True, env=env, cwd=d, timeout=600)
+        dt = time.perf_counter() - t0
+    if r.returncode != 0:
+        out["reference_error"] = r.stderr[-500:]
+        return out
+    out["reference_s"] = dt
+    out["reference_value"] = case["n_real"] * nsteps / dt
+    out["port_over_reference"] = out["port_value"] / out["reference_value"]
+    return out
 
 
 def cpu_baseline(case, seconds_target=20.0):
     """The oracle (C restatement of the reference path, OpenMP) on the host,
-    over a bounded number of full steps of the same workload."""
+    over a bounded number of full steps of the same workload, on every core
+    the job may use; beside it, the compiled reference and the port timed on
+    the reference's own case as a calibration."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    info = host_cpu()
+    threads = baseline_threads(info)
     oracle.set_threads(threads)
     sim = oracle.OracleSim(case)
     t0 = time.perf_counter()
@@ -187,11 +268,18 @@ def cpu_baseline(case, seconds_target=20.0):
     t2 = time.perf_counter()
     sim.run(nsteps)
     t3 = time.perf_counter()
-    return dict(value=case["n_real"] * nsteps / (t3 - t2), unit="particle-steps/s",
-                cores=threads, kind="port",
-                sample=f"the full workload ({case['n_real']} fluid + {case['n_dummy']} boundary), "
-                       f"1 warm-up + {nsteps} timed leap-frog steps, oracle/sph_oracle.c with "
-                       f"{threads} OpenMP threads")
+    del sim
+    out = dict(value=case["n_real"] * nsteps / (t3 - t2), unit="particle-steps/s",
+               cores=threads, kind="port",
+               sample=f"the full workload ({case['n_real']} fluid + {case['n_dummy']} boundary), "
+                      f"1 warm-up + {nsteps} timed leap-frog steps, oracle/sph_oracle.c with "
+                      f"{threads} OpenMP threads",
+               host=info)
+    try:
+        out["calibration"] = reference_calibration(threads)
+    except Exception as e:  # the port's figure stands on its own
+        out["calibration"] = {"error": repr(e)}
+    return out
 
 
 def serving_leg(args):
@@ -379,8 +467,19 @@ def main():
         out["mixed_rho_rel_err_vs_fp64"] = float(d)
     if "fp64" in res and "mixed" in res and args.accuracy_steps > 0:
         same = args.warmup + args.steps == args.accuracy_steps
-        out["accuracy"] = accuracy(case, args.accuracy_steps,
-                                   {k: res[k]["state"] for k in ("fp64", "mixed")} if same else None)
+        out["accuracy"], steady = accuracy(
+            case, args.accuracy_steps,
+            {k: res[k]["state"] for k in ("fp64", "mixed")} if same else None, warm=10)
+        if same:
+            steady = {k: 1e3 * res[k]["wall_s"] / args.steps for k in ("fp64", "mixed")}
+        w = args.warmup if same else 10
+        out["steady"] = {"steps": f"{w + 1}-{args.accuracy_steps}",
+                         "mixed_ms": steady["mixed"], "fp64_ms": steady["fp64"],
+                         "mixed_value": nf / (steady["mixed"] * 1e-3),
+                         "fp64_value": nf / (steady["fp64"] * 1e-3),
+                         "ratio": steady["fp64"] / steady["mixed"]}
+        log(f"steady over steps {out['steady']['steps']}: mixed {steady['mixed']:.3f} ms, "
+            f"fp64 {steady['fp64']:.3f} ms, ratio {out['steady']['ratio']:.3f}")
     if not args.no_pcie:
         out["pcie_inclusive"] = pcie_inclusive(case)
         log(f"drop-in single_step with host buffers: {out['pcie_inclusive']['ms_per_step']:.2f} ms/step")

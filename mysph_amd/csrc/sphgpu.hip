@@ -1941,12 +1941,15 @@ static void drop_graph(sphgpu_ctx* c) {
 // Capture kGraphSteps body steps (kernel nodes only: no memsets, no events).
 // The kernels take the uploaded Params and device arrays by value, and a body
 // step after the first has the same arguments every time, so the graph holds
-// until the next upload.
+// until the next upload. Capturing launches nothing, so it is done on the
+// first advance after an upload whatever its length (the steps are captured
+// as steps after the first), and no later call pays for it.
 template <int DIM>
 static int capture_body(sphgpu_ctx* c) {
     const int64_t step = c->step;
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->step = std::max<int64_t>(step, 2);
     for (int k = 0; k < kGraphSteps; ++k) {
         fast_body_step<DIM>(c, false);
         c->step++;
@@ -1969,13 +1972,13 @@ static int capture_body(sphgpu_ctx* c) {
 // is timed.
 template <int DIM>
 static int advance_fast(sphgpu_ctx* c, int nsteps) {
+    if (c->use_graph && !c->graph) {
+        int rc = capture_body<DIM>(c);
+        if (rc) return rc;
+    }
     launch_fast_head<DIM>(c, c->step == 1 ? 1 : 0);
     int body = nsteps - 1;
     if (c->use_graph && c->step > 1 && body >= kGraphSteps) {
-        if (!c->graph) {
-            int rc = capture_body<DIM>(c);
-            if (rc) return rc;
-        }
         const int reps = body / kGraphSteps;
         for (int r = 0; r < reps; ++r) {
             HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
