@@ -64,6 +64,7 @@ def lib():
                                     ctypes.c_double, dp, dp]
         L.oracle_derivatives.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
+        L.oracle_external_force.argtypes = [ctypes.c_void_p, ctypes.c_void_p, dp]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
@@ -159,6 +160,15 @@ class OracleSim:
         rc = lib().oracle_derivatives(ctypes.byref(self.cfg), ctypes.byref(self._arr))
         if rc != 0:
             raise RuntimeError(f"oracle_derivatives failed: {rc}")
+
+    def external_force(self) -> np.ndarray:
+        """The external force alone at the current state (BGGS, then
+        external_force), ntotal x dim."""
+        out = np.zeros((self.ntotal, self.dim))
+        rc = lib().oracle_external_force(ctypes.byref(self.cfg), ctypes.byref(self._arr), _dp(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle_external_force failed: {rc}")
+        return out
 
     def run(self, nsteps: int):
         rc = lib().oracle_run(ctypes.byref(self.cfg), ctypes.byref(self._arr), self.step,

@@ -14,7 +14,13 @@
 !!          deriv  one call of single_step (step 1), derivatives + state dump
 !!          run    leap_frog for nsteps steps, state dump
 !!          bggs   (case custom) BGGS on particles read from in.bin
+!!          extforce (case custom) BGGS, then external_force alone, on the
+!!                 full particle state read from in.bin (custom state format)
 !!   in.bin (optional, named cases, modes deriv/run): initial velocity override
+!!   case custom with modes deriv / run / extforce reads a whole particle
+!!   state and its switches from in.bin (see read_custom_state); deriv keeps
+!!   the slots past ntotal as dummy sources, run (leap_frog, which counts no
+!!   dummies unless it generates them) integrates real particles only.
 !!
 !! Output: a stream of records  name(char*8) kind(int32: 8=real64, 4=int32)
 !!         count(int64) data(count values).
@@ -48,7 +54,11 @@ program ref_driver
     call set_defaults()
 
     if (trim(case_name) == "custom") then
-        call run_custom_bggs()
+        if (trim(mode) == "bggs") then
+            call run_custom_bggs()
+        else
+            call run_custom_state()
+        end if
         stop
     end if
 
@@ -277,6 +287,92 @@ contains
         call BGGS(P(1:ntotal), P(1:n), skipItsSelf=.true.)
         call dump_pairs(ntotal)
     end subroutine run_custom_bggs
+
+    !! Custom state input: int32 dim, skf, ntotal, n, kpair, flags, pa_sph,
+    !! sle; char(16) nick; real64 dt; then real64 x(dim*n), v(dim*n) (both
+    !! particle-major), mass(n), rho(n), p(n), e(n), c(n), h(n); int32
+    !! type(n), state(n), boundary(n). flags: 1 viscosity, 2 gravity,
+    !! 4 ex_force, 8 arti_visc, 16 kernel correction, 32 sum_density,
+    !! 64 norm_dens, 128 arti_stress.
+    subroutine read_custom_state()
+        integer :: iu, d, hdr(8), flags
+        character(len=16) :: nk
+        real(8) :: dt
+        real(8), allocatable :: xx(:), vv(:), ms(:), rh(:), pp(:), ee(:), cc(:), hh(:)
+        integer, allocatable :: ty(:), st(:), bd(:)
+        open(newunit=iu, file=trim(in_path), access="stream", form="unformatted", status="old")
+        read(iu) hdr, nk, dt
+        Field%Dim = hdr(1); Config%skf = hdr(2); ntotal = hdr(3); n = hdr(4)
+        Field%pairNum = hdr(5); flags = hdr(6); Config%pa_sph = hdr(7); Config%sle = hdr(8)
+        Field%Maxn = n
+        Project%nick = trim(nk)
+        Config%delta_t = dt
+        Config%viscosity_w = iand(flags, 1) /= 0
+        Config%gravity_w = iand(flags, 2) /= 0
+        Config%ex_force_w = iand(flags, 4) /= 0
+        Config%arti_visc_w = iand(flags, 8) /= 0
+        Config%kernel_correciton_w = iand(flags, 16) /= 0
+        Config%sum_density_w = iand(flags, 32) /= 0
+        Config%norm_dens_w = iand(flags, 64) /= 0
+        Config%arti_stress_w = iand(flags, 128) /= 0
+        allocate(xx(Field%Dim*n), vv(Field%Dim*n), ms(n), rh(n), pp(n), ee(n), cc(n), hh(n))
+        allocate(ty(n), st(n), bd(n))
+        read(iu) xx, vv, ms, rh, pp, ee, cc, hh, ty, st, bd
+        close(iu)
+        call allocateParticleList(P, n, Field%Dim, Field%pairNum)
+        do i = 1, n
+            do d = 1, Field%Dim
+                P(i)%x(d) = xx((i-1)*Field%Dim + d)
+                P(i)%v(d) = vv((i-1)*Field%Dim + d)
+            end do
+            P(i)%Mass = ms(i); P(i)%Density = rh(i); P(i)%Pressure = pp(i)
+            P(i)%InternalEnergy = ee(i); P(i)%SoundSpeed = cc(i); P(i)%SmoothingLength = hh(i)
+            P(i)%Type = ty(i); P(i)%State = st(i); P(i)%Boundary = bd(i)
+        end do
+    end subroutine read_custom_state
+
+    subroutine run_custom_state()
+        use ex_force_m, only: external_force
+        real(8), allocatable :: exdvdt(:, :)
+        call read_custom_state()
+        select case (trim(mode))
+        case ("extforce")
+            !! singleStep.f90:84 then :116: the pair list of this state, then
+            !! the external force alone (it reads x, h, SoundSpeed and Type)
+            call BGGS(P(1:ntotal), P(1:n), skipItsSelf=.true.)
+            allocate(exdvdt(Field%Dim, n), source=0._8)
+            call external_force(ntotal, P, exdvdt)
+            open(newunit=u, file=trim(out_path), access="stream", form="unformatted", status="replace")
+            call put_i4(u, "ntotal", [ntotal])
+            call put_i4(u, "dim", [Field%Dim])
+            call put_i4(u, "count", P(1:ntotal)%neighborNum)
+            call put_r8(u, "exdvdt", reshape(exdvdt(:, 1:ntotal), [Field%Dim*ntotal]))
+            close(u)
+        case ("deriv")
+            allocate(Delta(Field%Maxn))
+            do i = 1, Field%Maxn
+                Delta(i)%Density = 0
+                Delta(i)%Energy  = 0
+                allocate(Delta(i)%Velocity(Field%Dim), source=0._8)
+            end do
+            allocate(aver_v(Field%Dim, Field%Maxn), source=0._8)
+            allocate(Shear(Field%Dim, Field%Dim, Field%Maxn), source=0._8)
+            allocate(dSdt(Field%Dim, Field%Dim, Field%Maxn), source=0._8)
+            Config%i_time_step = 1
+            ndummy = n - ntotal
+            nbuffer = 0
+            call single_step(ntotal, ndummy, nbuffer, P, Delta, aver_v, Shear, dSdt)
+            call dump_state(ntotal, n - ntotal)
+            call dump_delta(ntotal)
+        case ("run")
+            if (n /= ntotal) error stop "custom run: leap_frog integrates real particles only (n == ntotal)"
+            Config%max_time_step = nsteps
+            call time_integration(ntotal, P)
+            call dump_state(ntotal, 0)
+        case default
+            error stop "unknown mode for case custom"
+        end select
+    end subroutine run_custom_state
 
     subroutine put_r8(iu, name, a)
         integer, intent(in) :: iu

@@ -34,9 +34,12 @@
  *   The fluid_visc_skip_solid switch of internal_force has no reference
  *   counterpart (it drops a term the reference keeps); with it off, the
  *   routine is the reference's.
- *   external_force penalty and fluid-solid coupling branches: closed-form
- *   single-pair checks only (no reference case defines them; the gate case
- *   leaves its EOS sound speed uninitialised, eos.f90:35-40).
+ *   external_force penalty and fluid-solid coupling branches, the wall
+ *   force at the default range and the per-nick gravity variants: compiled
+ *   reference on seeded custom sets (external_force alone, nicks unknown,
+ *   db_gate, water_impact; tests/golden/custom_sets.py).
+ *   leap_frog's J2 cap (type 104): compiled reference on seeded type-104
+ *   blocks driven past the yield surface.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1071,6 +1074,19 @@ int oracle_derivatives(const sphgpu_config* cfg, const oracle_arrays* a) {
     or_state s;
     or_bind(a, &s);
     return or_single_step(cfg, &s, NULL);
+}
+
+/* The external force alone at the current state (singleStep.f90:84 then
+ * :116): BGGS of the state, then external_force into dvdt (ntotal x dim). */
+int oracle_external_force(const sphgpu_config* cfg, const oracle_arrays* a, double* dvdt) {
+    or_state s;
+    or_bind(a, &s);
+    or_pairs pr;
+    int rc = or_bggs(s.dim, cfg->skf, s.ntotal, s.n, s.x, s.h, s.state, &pr, NULL);
+    if (rc) return rc;
+    or_external_force(cfg, &s, &pr, dvdt);
+    or_pairs_free(&pr);
+    return 0;
 }
 
 /* nsteps of leap_frog starting at first_step (1 = fresh start). */

@@ -283,6 +283,53 @@ def custom_bggs():
     save("ref_bggs_custom", arrays, meta)
 
 
+def run_custom(mode, nsteps, s, dim, ntotal, flags, nick, dt, kpair=200):
+    """ref_driver on a custom particle state (tests/golden/custom_sets.py)."""
+    from tests.golden.custom_sets import write_driver_input
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "in.bin")
+        out = os.path.join(td, "out.bin")
+        write_driver_input(inp, s, dim, 1, ntotal, kpair, flags, nick, dt)
+        subprocess.run([DRIVER, "custom", mode, str(nsteps), out, inp], check=True, cwd=td,
+                       stdout=subprocess.DEVNULL, env={**os.environ, "OMP_NUM_THREADS": "1"})
+        return oracle.read_dump(out)
+
+
+def ext_force():
+    """external_force alone (externalForce.f90:7-174) on the custom sets:
+    penalty, coupling, wall force and the per-nick gravity variants."""
+    from tests.golden.custom_sets import external_force_sets
+    arrays, meta = {}, {"sets": []}
+    for name, nick, dim, flags, ntotal, s in external_force_sets():
+        d = run_custom("extforce", 0, s, dim, ntotal, flags, nick, 1e-4)
+        a = d["exdvdt"].reshape(-1, dim)
+        meta["sets"].append(name)
+        meta[f"{name}_nick"] = nick
+        meta[f"{name}_sha_exdvdt"] = sha(a)
+        meta[f"{name}_sha_count"] = sha(d["count"])
+        meta[f"{name}_sha_x"] = sha(s["x"])
+        meta[f"{name}_pairs"] = int(d["count"].sum())
+        arrays[f"{name}_exdvdt"] = a
+    save("ref_ext_force", arrays, meta)
+
+
+def j2_cap():
+    """leap_frog with the J2 cap of type 104 (timeIntegration.f90:149-166,
+    :190-205, :225-240) on seeded blocks; state after 1, 30 and 60 steps."""
+    from tests.golden.custom_sets import j2_sets
+    arrays, meta = {"_": np.zeros(1)}, {"sets": []}
+    for name, dim, flags, nsteps, dt, s in j2_sets():
+        meta["sets"].append(name)
+        meta[f"{name}_sha_x0"] = sha(s["x"])
+        nt = len(s["h"])
+        for n in (1, nsteps // 2, nsteps):
+            d = run_custom("run", n, s, dim, nt, flags, "unknown", dt)
+            a, m = flat(f"{name}_step{n}_", state_record(d, nt, 7))
+            arrays.update(a)
+            meta.update(m)
+    save("ref_j2_cap", arrays, meta)
+
+
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         oracle.build_ref(verbose=True)

@@ -203,3 +203,95 @@ def test_custom_bggs_window_misses_neighbours():
     d = np.sqrt(((x[:nt, None, :] - x[None, :, :]) ** 2).sum(-1))
     brute = (d < 2 * s["h"][:nt, None]).sum() - nt
     assert len(nbr) < brute
+
+
+# ---------------------------------------------------------------- custom sets
+# external_force's penalty and coupling branches and the J2 cap of leap_frog
+# run in no shipped reference case with defined values; the compiled
+# reference runs them on seeded sets (tests/golden/custom_sets.py).
+
+def test_external_force_branches_bit_exact():
+    from tests.golden.custom_sets import as_case, external_force_sets, nick_config
+    z, meta = load("ref_ext_force")
+    names = []
+    for name, nick, dim, flags, ntotal, s in external_force_sets():
+        assert sha(s["x"]) == meta[f"{name}_sha_x"], f"{name}: inputs changed"
+        cfg = nick_config(dim, nick, flags, 1e-4, s["x"])
+        o = oracle.OracleSim(as_case(s, ntotal, dim, cfg))
+        a = o.external_force()
+        if sha(a) != meta[f"{name}_sha_exdvdt"]:
+            ref = z[f"{name}_exdvdt"]
+            pytest.fail(f"{name}: external force not bit-identical to the reference "
+                        f"(max rel diff {np.max(np.abs(a - ref)) / np.max(np.abs(ref)):.3e})")
+        names.append(name)
+    assert names == meta["sets"]
+
+
+def test_external_force_sets_exercise_every_branch():
+    """The sets are only a pin if the branches fire: count the pairs that
+    take the penalty (pe >= 1), the coupling (chi and factor_c nonzero, each
+    factor_c piece), the wall force (r < r0) and solid-skipped gravity."""
+    from tests.golden.custom_sets import as_case, external_force_sets, nick_config
+    hits = dict(penalty=0, coupling=0, fc_flat=0, fc_mid=0, fc_tail=0, wall=0, skip_g=0)
+    for name, nick, dim, flags, ntotal, s in external_force_sets():
+        cfg = nick_config(dim, nick, flags, 1e-4, s["x"])
+        off, nbr, _, _, _ = oracle.bggs(dim, 1, ntotal, s["x"], s["h"], s["state"])
+        t = s["type"]
+        for i in range(ntotal):
+            for j in nbr[off[i]:off[i + 1]]:
+                r = np.linalg.norm(s["x"][i] - s["x"][j])
+                if t[i] > 0 and t[j] < 0 and r < cfg.lj_r0:
+                    hits["wall"] += 1
+                if t[i] != t[j] and t[i] > 0 and t[j] > 0:
+                    hm = 0.5 * (s["h"][i] + s["h"][j])
+                    if t[i] < 100 and t[j] < 100:
+                        hits["penalty"] += hm / r >= 1
+                    elif not (t[i] > 100 and t[j] > 100) and r < cfg.couple_delta:
+                        hits["coupling"] += 1
+                        eta = r / 0.75 / hm
+                        hits["fc_flat"] += eta <= 2 / 3
+                        hits["fc_mid"] += 2 / 3 < eta <= 1
+                        hits["fc_tail"] += 1 < eta < 2
+        if cfg.gravity_mode == 1:
+            hits["skip_g"] += int(np.sum(t[:ntotal] > 100))
+    assert all(v > 0 for v in hits.values()), hits
+
+
+def test_j2_cap_bit_exact():
+    """leap_frog with type-104 solids whose deviatoric stress reaches the
+    yield surface: the oracle's J2 cap against the reference's."""
+    from tests.golden.custom_sets import as_case, j2_config, j2_sets
+    z, meta = load("ref_j2_cap")
+    for name, dim, flags, nsteps, dt, s in j2_sets():
+        assert sha(s["x"]) == meta[f"{name}_sha_x0"]
+        nt = len(s["h"])
+        o = oracle.OracleSim(as_case(s, nt, dim, j2_config(dim, dt, flags)))
+        done = 0
+        for n in (1, nsteps // 2, nsteps):
+            o.run(n - done)
+            done = n
+            for k in ("x", "v", "rho", "p", "stress"):
+                key = f"{name}_step{n}_@{k}"
+                a = o[k][:nt].reshape(nt, -1)
+                check_field(meta, z, key, a, 7)
+
+
+def test_j2_cap_engages():
+    """The J2 sets reach the cap: with the yield stress raised, the oracle's
+    deviatoric stress at the end is larger than with the reference's."""
+    from tests.golden.custom_sets import as_case, j2_config, j2_sets
+    for name, dim, flags, nsteps, dt, s in j2_sets():
+        nt = len(s["h"])
+        out = []
+        for sy in (None, 1e12):
+            cfg = j2_config(dim, dt, flags)
+            if sy:
+                cfg.mat[0].sigma_y = sy
+            o = oracle.OracleSim(as_case(s, nt, dim, cfg))
+            o.run(nsteps)
+            out.append(np.sqrt(np.sum(o["shear"][:nt] ** 2, axis=(1, 2))))
+        capped, free = out
+        ys = np.sqrt(3e8 ** 2 / 3)     # sqrt(J2) on the yield surface
+        assert np.max(capped) <= ys * (1 + 1e-12)
+        over = np.sum(free > 1.05 * ys)
+        assert over > nt // 10, f"{name}: the cap engages for {over} of {nt} particles only"

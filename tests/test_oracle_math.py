@@ -193,29 +193,47 @@ def test_wall_repulsion_single_pair():
     assert np.allclose(s1["dvdt"][0] - s0["dvdt"][0], expect, rtol=1e-12)
 
 
-def test_fluid_solid_coupling_single_pair():
-    """externalForce.f90:142-170 for one fluid target and one solid neighbour."""
+def _coupling_pair(r, c_fluid=20.0, c_solid=20.0, delta=0.25, h=0.12):
+    """Coupling force on each of a fluid and a solid particle at distance r
+    along x (both real, so each is the other's target), everything else off."""
     case = blob(n=4)
-    case["x"] = np.array([[0.0, 0.0], [0.1, 0.0], [5, 5], [6, 6]])
+    case["x"] = np.array([[0.0, 0.0], [r, 0.0], [5, 5], [6, 6]])
     case["type"] = np.array([2, 102, 2, 2], np.int32)
     case["v"][:] = 0
+    case["h"][:] = h
+    case["c"][:] = [c_fluid, c_solid, 20.0, 20.0]
     cfg = case["config"].copy()
     cfg.artificial_viscosity = 0
     cfg.external_force = 1
-    cfg.couple_delta = 0.25
-    cfg.add_material(102, C.EOS_LINEAR, 1000.0, 1e4, 1.0, eta=1e5)
-    s = oracle.OracleSim(case, cfg)
-    s.derivatives()
-    cfg0 = cfg.copy()
-    cfg0.external_force = 0
-    s0 = oracle.OracleSim(case, cfg0)
-    s0.derivatives()
-    r, h = 0.1, 0.12
-    eta = r / 0.75 / h
-    fc = 2 * eta - 1.5 * eta * eta if eta <= 1 else 0.5 * (2 - eta) ** 2
-    chi = 1 - r / 0.25
-    expect = float(np.float32(0.01)) * 20.0 ** 2 * chi * fc * np.array([-0.1, 0.0]) / r ** 2
-    assert np.allclose(s["dvdt"][0] - s0["dvdt"][0], expect, rtol=1e-12)
+    cfg.gravity = 0
+    cfg.couple_delta = delta
+    return oracle.OracleSim(case, cfg).external_force()[:2]
+
+
+def test_fluid_solid_coupling_invariants():
+    """externalForce.f90:142-170, through properties a restatement error
+    breaks: the force lies along dx and repels; it scales with the target's
+    own c^2 and not the neighbour's; equal sound speeds give equal and
+    opposite forces; it is continuous across the branch points of its
+    kernel-like factor (eta = 2/3, 1) and vanishes beyond delta and at
+    eta = 2. (The bits are pinned to the compiled reference by
+    test_oracle_golden.test_external_force_branches_bit_exact.)"""
+    f = _coupling_pair(0.1)
+    assert f[0, 1] == 0 and f[0, 0] < 0 and f[1, 0] > 0
+    assert np.array_equal(f[0], -f[1])
+    g = _coupling_pair(0.1, c_fluid=40.0)
+    assert np.allclose(g[0], 4 * f[0], rtol=1e-15) and np.array_equal(g[1], f[1])
+    h = 0.12
+    for eta_b in (2.0 / 3.0, 1.0):
+        rb = eta_b * 0.75 * h
+        lo, hi = _coupling_pair(rb * (1 - 1e-9)), _coupling_pair(rb * (1 + 1e-9))
+        assert abs(hi[0, 0] / lo[0, 0] - 1) < 1e-7, eta_b
+    # neighbours (r < 2h) beyond delta: chi = 0; just inside: not 0
+    assert _coupling_pair(0.2500001, h=0.2)[0, 0] == 0
+    assert _coupling_pair(0.2499, h=0.2)[0, 0] != 0
+    # eta = r / 0.75 / h past 2: factor 0; just below: not 0
+    assert _coupling_pair(0.185, delta=0.5)[0, 0] == 0
+    assert _coupling_pair(0.175, delta=0.5)[0, 0] != 0
 
 
 # --------------------------------------------------------------- integrator
