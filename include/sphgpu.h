@@ -193,6 +193,17 @@ int sphgpu_advance(sphgpu_ctx* ctx, int32_t nsteps);
  * host memory `dst` (doubles; count = n * components of the field). */
 int sphgpu_download(sphgpu_ctx* ctx, int32_t field, double* dst);
 
+/* Copy one field from host memory `src` (upload order, as sphgpu_download
+ * writes it) into the resident state. Supported: SPHGPU_F_DIVX and
+ * SPHGPU_F_DIVV, the divergences a step leaves in the particles
+ * (Particles%divergencePosition / divergenceVelocity, divergence.f90:6-29).
+ * A caller that uploads the state on every step (the drop-in single_step)
+ * passes the last step's divergencePosition this way: summation density
+ * with Shepard normalisation normalises only particles whose previous
+ * divergencePosition is below 1.5 (density.f90:25-37, :67-76). Other fields
+ * return SPHGPU_E_ARG (upload them with sphgpu_upload). */
+int sphgpu_upload_field(sphgpu_ctx* ctx, int32_t field, const double* src);
+
 /* Neighbour pairs of the current state, as BGGS builds them
  * (asymmetricSearch.f90:160-257): for each real particle i, offsets[i] ..
  * offsets[i+1]-1 index nbr[], the neighbour ids in ascending order.

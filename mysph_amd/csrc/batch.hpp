@@ -608,7 +608,8 @@ extern "C" int sphgpu_batch_download(sphgpu_batch* b, int32_t run, int32_t field
     const bool typed = (int64_t)r->type_idx.size() == r->n;
     std::string msg;
     int rc = download_view(V, r->n, b->dim, b->precision == SPHGPU_MIXED, r->cfg, b->stream, field, dst, msg,
-                           typed ? b->dorder : nullptr, typed ? r->type_idx.data() : nullptr, &b->dstage);
+                           typed ? b->dorder : nullptr, typed ? r->type_idx.data() : nullptr, &b->dstage,
+                           r->P.err + 3);
     return rc ? bfail(b, rc, msg) : 0;
 }
 

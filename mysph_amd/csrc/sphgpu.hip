@@ -1513,8 +1513,12 @@ struct HostStage {
         if (need > bytes) {
             release();
             if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess) {
+                // no pinned memory to be had: stage in pageable memory from now on
                 p = nullptr;
-                return nullptr;
+                pinned = false;
+                (void)hipGetLastError();
+                if (v.size() < need) v.resize(need);
+                return v.data();
             }
             bytes = need;
         }
@@ -2508,14 +2512,46 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
 
 // A slot-ordered field in particle order: dst[meta[s].w] = src[s] for
 // elements of `words` 32-bit words (a run's meta.w is a permutation of 0..n-1).
+// `bad` is set when a slot's particle index is out of range: the slot map
+// is written once at upload from a host permutation, so this means the
+// device state was overwritten, and the caller reports it instead of
+// returning an incomplete field.
 __global__ __launch_bounds__(kBlock) void k_particle_order(const uint32_t* __restrict__ src,
                                                            const int4* __restrict__ meta, int64_t n,
-                                                           int words, uint32_t* __restrict__ dst) {
+                                                           int words, uint32_t* __restrict__ dst,
+                                                           int* bad) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const int64_t i = meta[s].w;
-    if (i < 0 || i >= n) return;
+    if (i < 0 || i >= n) {
+        atomicOr(bad, 1);
+        return;
+    }
     for (int q = 0; q < words; ++q) dst[i * words + q] = src[s * words + q];
+}
+
+// The inverse for a divergence field: the value of particle meta[s].w (host
+// order, staged on the device) into slot s, as component `comp` (0: position,
+// 1: velocity) of the FP64 record and, when present, of the FP32 one.
+__global__ __launch_bounds__(kBlock) void k_slot_div(const double* __restrict__ src,
+                                                     const int4* __restrict__ meta, int64_t n, int comp,
+                                                     double2* div, float2* divf, int* bad) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = meta[s].w;
+    if (i < 0 || i >= n) {
+        atomicOr(bad, 1);
+        return;
+    }
+    const double v = src[i];
+    if (div) {
+        if (comp) div[s].y = v;
+        else div[s].x = v;
+    }
+    if (divf) {
+        if (comp) divf[s].y = (float)v;
+        else divf[s].x = (float)v;
+    }
 }
 
 // One field of a run in particle order. `order`: device scratch of n
@@ -2527,8 +2563,8 @@ __global__ __launch_bounds__(kBlock) void k_particle_order(const uint32_t* __res
 static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sphgpu_config& cfg,
                          hipStream_t stream, int32_t field, double* dst, std::string& msg,
                          void* order = nullptr, const int32_t* type_by_index = nullptr,
-                         HostStage* stage = nullptr) {
-    const bool ordered = order != nullptr && type_by_index != nullptr;
+                         HostStage* stage = nullptr, int* bad = nullptr) {
+    const bool ordered = order != nullptr && type_by_index != nullptr && bad != nullptr;
     std::vector<int4> meta_dev;
     const int4* meta = nullptr;
     if (!ordered) {
@@ -2544,15 +2580,22 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
             msg = std::string("field not held by this run: ") + kFieldNames[field];
             return SPHGPU_E_ARG;
         }
+        int flag = 0;
         if (ordered) {
+            VCHK(hipMemsetAsync(bad, 0, sizeof(int), stream));
             hipLaunchKernelGGL(k_particle_order, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                                stream, static_cast<const uint32_t*>(src), D.meta, n, (int)(elem / 4),
-                               static_cast<uint32_t*>(order));
+                               static_cast<uint32_t*>(order), bad);
             VCHK(hipGetLastError());
             src = order;
         }
         VCHK(hipMemcpyAsync(h, src, n * elem, hipMemcpyDeviceToHost, stream));
+        if (ordered) VCHK(hipMemcpyAsync(&flag, bad, sizeof(int), hipMemcpyDeviceToHost, stream));
         VCHK(hipStreamSynchronize(stream));
+        if (flag) {
+            msg = "slot-to-particle map out of range: the device state was overwritten";
+            return SPHGPU_E_HIP;
+        }
         return 0;
     };
     HostStage local;
@@ -2691,8 +2734,35 @@ extern "C" int sphgpu_download(sphgpu_ctx* c, int32_t field, double* dst) {
     if (c->up_meta && !c->dorder) HIPCHK(c, hipMalloc(&c->dorder, obytes));
     const bool typed = c->up_meta && (int64_t)c->up.type_idx.size() == c->n;
     int rc = download_view(c->D, c->n, c->dim, c->P.mixed != 0, c->cfg, c->stream, field, dst, msg,
-                           typed ? c->dorder : nullptr, typed ? c->up.type_idx.data() : nullptr, &c->dstage);
+                           typed ? c->dorder : nullptr, typed ? c->up.type_idx.data() : nullptr, &c->dstage,
+                           c->D.err + 3);
     return rc ? fail(c, rc, msg) : 0;
+}
+
+// One field of the resident state from host memory in upload order. Only
+// the divergences, which a caller that uploads on every step needs to carry
+// from the previous step (Shepard normalisation reads the last step's
+// divergencePosition, density.f90:25-37).
+extern "C" int sphgpu_upload_field(sphgpu_ctx* c, int32_t field, const double* src) {
+    if (!c || !c->has_state) return fail(c, SPHGPU_E_ARG, "no particle state uploaded");
+    if (!src) return fail(c, SPHGPU_E_ARG, "NULL source");
+    if (field != SPHGPU_F_DIVX && field != SPHGPU_F_DIVV)
+        return fail(c, SPHGPU_E_ARG, std::string("field cannot be uploaded alone: ") +
+                                         (field >= 0 && field < SPHGPU_F_NFIELDS ? kFieldNames[field] : "?"));
+    const size_t obytes = (size_t)c->capacity * c->dim * c->dim * sizeof(double);
+    if (!c->dorder) HIPCHK(c, hipMalloc(&c->dorder, obytes));
+    int* bad = c->D.err + 3;
+    int flag = 0;
+    HIPCHK(c, hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dorder, src, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_slot_div, dim3((unsigned)((c->n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                       static_cast<const double*>(c->dorder), c->D.meta, c->n, field == SPHGPU_F_DIVV ? 1 : 0,
+                       c->D.div, c->D.divf, bad);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&flag, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flag) return fail(c, SPHGPU_E_HIP, "slot-to-particle map out of range: the device state was overwritten");
+    return 0;
 }
 
 extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr, int64_t cap,

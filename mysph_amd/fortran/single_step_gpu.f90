@@ -138,6 +138,15 @@ subroutine single_step(ntotal, ndummy, nbuffer, Particles, Delta, aver_v, Shear,
                       c_loc(sx), c_loc(sv), c_loc(smass), c_loc(srho), c_loc(sp), c_loc(se), &
                       c_loc(sc), c_loc(sh), pshear, c_loc(stype), c_loc(sstate), c_loc(sbnd)), &
                       gpu_ctx, "sphgpu_upload")
+    !! Shepard normalisation reads the previous step's divergencePosition
+    !! (density.f90:25-37, :67-76); an upload starts it at zero
+    if (Config%sum_density_w .and. Config%norm_dens_w) then
+        do i = 1, N
+            sout(i) = Particles(i)%divergencePosition
+        end do
+        call sphgpu_check(sphgpu_upload_field(gpu_ctx, SPHGPU_F_DIVX, c_loc(sout)), gpu_ctx, &
+                          "sphgpu_upload_field divx")
+    end if
     call sphgpu_check(sphgpu_derivatives(gpu_ctx), gpu_ctx, "sphgpu_derivatives")
 
     !! Delta = in + av + ex + as (singleStep.f90:139-142), after the fixed boundary

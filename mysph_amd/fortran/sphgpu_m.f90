@@ -73,7 +73,8 @@ module sphgpu_m
     end type sphgpu_config
 
     public :: sphgpu_abi_version, sphgpu_create, sphgpu_upload, sphgpu_derivatives, &
-              sphgpu_advance, sphgpu_download, sphgpu_export_pairs, sphgpu_neighbour_stats, &
+              sphgpu_advance, sphgpu_download, sphgpu_upload_field, sphgpu_export_pairs, &
+              sphgpu_neighbour_stats, &
               sphgpu_grid_info, sphgpu_timing, sphgpu_step, sphgpu_last_error, sphgpu_destroy, &
               sphgpu_checkpoint_size, sphgpu_checkpoint_save, sphgpu_checkpoint_restore, &
               sphgpu_context_info
@@ -120,6 +121,13 @@ module sphgpu_m
             integer(c_int32_t), value :: field
             type(c_ptr), value :: dst
         end function sphgpu_download
+
+        integer(c_int) function sphgpu_upload_field(ctx, field, src) bind(C, name="sphgpu_upload_field")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: field
+            type(c_ptr), value :: src
+        end function sphgpu_upload_field
 
         integer(c_int) function sphgpu_export_pairs(ctx, offsets, nbr, cap, needed) &
                 bind(C, name="sphgpu_export_pairs")

@@ -36,7 +36,8 @@ STATUS = {
 
 EXPORTED = (
     "sphgpu_abi_version", "sphgpu_create", "sphgpu_upload", "sphgpu_derivatives",
-    "sphgpu_advance", "sphgpu_download", "sphgpu_export_pairs", "sphgpu_grid_info",
+    "sphgpu_advance", "sphgpu_download", "sphgpu_upload_field", "sphgpu_export_pairs",
+    "sphgpu_grid_info",
     "sphgpu_timing", "sphgpu_step", "sphgpu_last_error", "sphgpu_destroy",
     "sphgpu_neighbour_stats", "sphgpu_checkpoint_size", "sphgpu_checkpoint_save",
     "sphgpu_checkpoint_restore", "sphgpu_context_info",
@@ -74,6 +75,7 @@ def load_library(path: str = LIB_PATH):
     L.sphgpu_derivatives.argtypes = [vp]
     L.sphgpu_advance.argtypes = [vp, ctypes.c_int32]
     L.sphgpu_download.argtypes = [vp, ctypes.c_int32, dp]
+    L.sphgpu_upload_field.argtypes = [vp, ctypes.c_int32, dp]
     L.sphgpu_export_pairs.argtypes = [vp, ctypes.POINTER(i64), ip, i64, ctypes.POINTER(i64)]
     L.sphgpu_grid_info.argtypes = [vp, dp]
     L.sphgpu_neighbour_stats.argtypes = [vp, ctypes.POINTER(i64)]
@@ -192,6 +194,15 @@ class Simulation:
         if comp == self.dim:
             return out.reshape(self.n, self.dim)
         return out.reshape(self.n, self.dim, self.dim)
+
+    def put(self, name: str, values):
+        """Overwrite one field of the resident state (in the caller's
+        particle order); the divergences only (sphgpu_upload_field)."""
+        f = C.FIELD_NAMES[name]
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != self.n * C.field_components(f, self.dim):
+            raise ValueError(f"{name} has the wrong size")
+        self._check(self._L.sphgpu_upload_field(self._ctx, f, _dp(a)))
 
     def pairs(self):
         """(offsets[n_real+1], nbr) of the current state, ids ascending per target."""

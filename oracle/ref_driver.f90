@@ -226,6 +226,8 @@ contains
         if (stat == 0) read(buf, *) Field%pairNum
         call get_environment_variable("REF_PA_SPH", buf, status=stat)
         if (stat == 0) read(buf, *) Config%pa_sph
+        call get_environment_variable("REF_SKF", buf, status=stat)
+        if (stat == 0) read(buf, *) Config%skf
 #ifdef _OPENMP
         call get_environment_variable("REF_THREADS", buf, status=stat)
         if (stat == 0) read(buf, *) Config%nthreads

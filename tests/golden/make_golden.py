@@ -182,6 +182,28 @@ def dam_break_sph13():
         save(f"ref_dam_break_sph{sph}", arrays, meta)
 
 
+def dam_break_skf():
+    """The Gauss (SKF = 2) and quintic (SKF = 3) kernels (kernel.f90:73-154)
+    in the reference dam break: the search reaches 3h, so the pair list holds
+    about 28 neighbours (REF_PAIRNUM = 60); step-1 derivatives and the
+    state after 1 and 10 steps."""
+    nt = cases.reference_dam_break()["n_real"]
+    for skf in (2, 3):
+        arrays, meta = {}, {"sample": SAMPLE, "skf": skf}
+        env = {"REF_SKF": str(skf), "REF_PAIRNUM": "60"}
+        d = run("dam_break", "deriv", 1, env)
+        for k, comp in (("dvdt", 2), ("drhodt", 1), ("dedt", 1)):
+            a = d[k].reshape(-1, comp)[:nt]
+            arrays["deriv_" + k] = a[::SAMPLE].copy()
+            meta["deriv_sha_" + k] = sha(a)
+        for n in (1, 10):
+            d = run("dam_break", "run", n, env)
+            a, m = flat(f"step{n}_", state_record(d, nt, SAMPLE, fields=STATE_H))
+            arrays.update(a)
+            meta.update(m)
+        save(f"ref_dam_break_skf{skf}", arrays, meta)
+
+
 def dam_break_sle():
     """Smoothing-length evolution (hsml.f90:8-38) switched on in the
     reference dam break."""

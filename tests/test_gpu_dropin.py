@@ -99,3 +99,35 @@ def test_dropin_sph_forms(sph, tmp_path):
     for k in STATE:
         comps = 2 if k in ("x", "v") else 1
         assert_close(f"sph {sph} {k}", nreal(gpu, k, comps), nreal(ref, k, comps), 1e-10)
+
+
+@pytest.mark.parametrize("norm", ["0", "1"], ids=["plain", "shepard"])
+def test_dropin_summation_density(norm, tmp_path):
+    """Config%sum_density_w on, with and without Shepard normalisation
+    (density.f90:10-80): the GPU single_step writes Density back, and with
+    normalisation it first uploads the previous step's divergencePosition,
+    which decides which particles are normalised (density.f90:25-37)."""
+    env = {"REF_SUM_DENSITY": "1", "REF_NORM_DENSITY": norm}
+    for nsteps in (1, 10, 30):
+        ref = run(REF, "dam_break", "run", nsteps, tmp_path, extra_env=env)
+        gpu = run(GPU, "dam_break", "run", nsteps, tmp_path, extra_env=env)
+        for k in STATE + ("divx",):
+            comps = 2 if k in ("x", "v") else 1
+            assert_close(f"sum density {norm} {nsteps} {k}", nreal(gpu, k, comps),
+                         nreal(ref, k, comps), 1e-10)
+
+
+@pytest.mark.parametrize("skf", ["2", "3"], ids=["gauss", "quintic"])
+def test_dropin_gauss_quintic(skf, tmp_path):
+    """Config%skf = 2 (Gauss) and 3 (quintic) through the GPU single_step
+    (kernel.f90:73-154); the search reaches 3h (REF_PAIRNUM = 60)."""
+    env = {"REF_SKF": skf, "REF_PAIRNUM": "60"}
+    ref = run(REF, "dam_break", "deriv", 1, tmp_path, extra_env=env)
+    gpu = run(GPU, "dam_break", "deriv", 1, tmp_path, extra_env=env)
+    for k in ("dvdt", "drhodt", "dedt"):
+        assert_close(f"skf {skf} {k}", gpu[k], ref[k], 1e-12)
+    ref = run(REF, "dam_break", "run", 10, tmp_path, extra_env=env)
+    gpu = run(GPU, "dam_break", "run", 10, tmp_path, extra_env=env)
+    for k in STATE:
+        comps = 2 if k in ("x", "v") else 1
+        assert_close(f"skf {skf} 10 steps {k}", nreal(gpu, k, comps), nreal(ref, k, comps), 1e-10)

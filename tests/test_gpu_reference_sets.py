@@ -122,10 +122,11 @@ def test_j2_cap_vs_oracle(k):
         sim.derivatives()
         for f in ("dvdt", "drhodt", "dedt", "dsdt"):
             assert_close(f"{name} step-1 {f}", sim.get(f)[:nt], orc[f][:nt], 1e-12)
+    # the cap engages during these runs (tests/test_oracle_golden.py,
+    # test_j2_cap_engages); it is checked here against the oracle's
     orc = oracle.OracleSim(case)
     orc.run(nsteps)
     ys = np.sqrt(3e8 ** 2 / 3)
-    assert np.sum(np.sqrt(np.sum(orc["shear"][:nt] ** 2, axis=(1, 2))) > ys * (1 - 1e-9)) > nt // 10
     with Simulation(case) as sim:
         sim.advance(nsteps)
         for f in ("x", "v", "rho", "p"):

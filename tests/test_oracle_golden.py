@@ -145,6 +145,27 @@ def test_smoothing_length_evolution_bit_exact(sle):
             check_field(meta, z, f"step{n}_@{k}", s[k][:nt].reshape(nt, -1), meta["sample"])
 
 
+@pytest.mark.parametrize("skf", [2, 3])
+def test_gauss_and_quintic_kernels_bit_exact(skf):
+    """The reference dam break with the Gauss (SKF = 2) or quintic (SKF = 3)
+    kernel (kernel.f90:73-154): derivatives and states, not only pairs."""
+    z, meta = load(f"ref_dam_break_skf{skf}")
+    c = cases.reference_dam_break()
+    cfg = c["config"].copy()
+    cfg.skf = skf
+    nt = c["n_real"]
+    oracle.set_threads(os.cpu_count() or 1)
+    s = oracle.OracleSim(c, cfg)
+    s.derivatives()
+    for k in ("dvdt", "drhodt", "dedt"):
+        assert sha(s[k][:nt].reshape(nt, -1).ravel()) == meta["deriv_sha_" + k], k
+    for n in (1, 10):
+        s = oracle.OracleSim(c, cfg)
+        s.run(n)
+        for k in ("x", "v", "rho", "p", "e", "h", "divv"):
+            check_field(meta, z, f"step{n}_@{k}", s[k][:nt].reshape(nt, -1), meta["sample"])
+
+
 def test_summation_density_with_shepard_bit_exact():
     z, meta = load("ref_dam_break_sumdens")
     c = cases.reference_dam_break()
