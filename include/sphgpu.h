@@ -212,6 +212,28 @@ int sphgpu_upload_field(sphgpu_ctx* ctx, int32_t field, const double* src);
 int sphgpu_export_pairs(sphgpu_ctx* ctx, int64_t* offsets, int32_t* nbr,
                         int64_t cap, int64_t* needed);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (serving loops that forbid it, harnesses that run under an
+ * allocation ban): the pair set of sphgpu_export_pairs for a particle state
+ * held in the caller's own device memory, with no context and no
+ * allocation. All inputs and outputs are DEVICE pointers: x (n * dim,
+ * particle-major), h (n), state (n, or NULL: all 0); offsets (n_real + 1)
+ * and nbr (cap ids) receive the CSR result, ids ascending per target;
+ * *needed (host) receives the pair count. Particles i < n_real are the
+ * targets, all n are sources; skf picks the reach (2h cubic, 3h Gauss and
+ * quintic). Work runs on `stream` (a hipStream_t, NULL for the null
+ * stream), which the call synchronises before it returns.
+ * Limit: n <= 1048576 particles, and a reference grid of at most 1114112
+ * cells (SPHGPU_E_CAPACITY beyond either). The library's scratch for this
+ * call is one static device array of about 18 MB per GPU, held until the
+ * process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race and must be serialised by the caller. cap too small:
+ * SPHGPU_E_PAIR_CAPACITY with *needed set and offsets written. Failure
+ * message: sphgpu_last_error(NULL). */
+int sphgpu_pairs_noalloc(int32_t dim, int32_t skf, int64_t n_real, int64_t n, const double* x,
+                         const double* h, const int32_t* state, int64_t* offsets, int32_t* nbr,
+                         int64_t cap, int64_t* needed, void* stream);
+
 /* Neighbour statistics of the last neighbour search, as print_statistics
  * reports them (src/NNPS/nnps.f90:377-416): out[0] total pairs, out[1] max
  * and out[2] min neighbours per real particle, out[3] real particles with
@@ -252,7 +274,7 @@ int sphgpu_context_info(sphgpu_ctx* ctx, sphgpu_config* cfg_out, int64_t* counts
 
 /* Message of the last failed call on ctx; with ctx == NULL, the message of
  * the last failed sphgpu_create or sphgpu_checkpoint_restore on the calling
- * thread. */
+ * thread (or of sphgpu_pairs_noalloc). */
 const char* sphgpu_last_error(sphgpu_ctx* ctx);
 
 void sphgpu_destroy(sphgpu_ctx* ctx);

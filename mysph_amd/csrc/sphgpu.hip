@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -2878,3 +2879,4 @@ extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
 
 #include "checkpoint.hpp"
 #include "batch.hpp"
+#include "noalloc.hpp"

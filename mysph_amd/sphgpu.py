@@ -37,7 +37,7 @@ STATUS = {
 EXPORTED = (
     "sphgpu_abi_version", "sphgpu_create", "sphgpu_upload", "sphgpu_derivatives",
     "sphgpu_advance", "sphgpu_download", "sphgpu_upload_field", "sphgpu_export_pairs",
-    "sphgpu_grid_info",
+    "sphgpu_pairs_noalloc", "sphgpu_grid_info",
     "sphgpu_timing", "sphgpu_step", "sphgpu_last_error", "sphgpu_destroy",
     "sphgpu_neighbour_stats", "sphgpu_checkpoint_size", "sphgpu_checkpoint_save",
     "sphgpu_checkpoint_restore", "sphgpu_context_info",
@@ -76,6 +76,8 @@ def load_library(path: str = LIB_PATH):
     L.sphgpu_advance.argtypes = [vp, ctypes.c_int32]
     L.sphgpu_download.argtypes = [vp, ctypes.c_int32, dp]
     L.sphgpu_upload_field.argtypes = [vp, ctypes.c_int32, dp]
+    L.sphgpu_pairs_noalloc.argtypes = [ctypes.c_int32, ctypes.c_int32, i64, i64, vp, vp, vp, vp, vp,
+                                       i64, ctypes.POINTER(i64), vp]
     L.sphgpu_export_pairs.argtypes = [vp, ctypes.POINTER(i64), ip, i64, ctypes.POINTER(i64)]
     L.sphgpu_grid_info.argtypes = [vp, dp]
     L.sphgpu_neighbour_stats.argtypes = [vp, ctypes.POINTER(i64)]
