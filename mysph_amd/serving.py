@@ -189,11 +189,12 @@ class Server:
     and so do those behind it), advance every resident run by up to `chunk`
     steps, and retire the runs that finished (or failed) in that chunk."""
 
-    def __init__(self, batch, chunk: int = 50, tuner: Tuner | None = None, keep=()):
+    def __init__(self, batch, chunk: int = 50, tuner: Tuner | None = None, keep=(), keep_ids=None):
         self.batch = batch
         self.chunk = int(chunk)
         self.tuner = tuner
         self.keep = tuple(keep)          # fields to download from each finished run
+        self.keep_ids = None if keep_ids is None else set(keep_ids)   # only these requests
         self.advance_s = 0.0
         self.chunks = 0
 
@@ -235,7 +236,8 @@ class Server:
                 if state == 0:
                     continue
                 req, admitted = active.pop(h)
-                fields = {k: self.batch.get(h, k) for k in self.keep}
+                fields = ({k: self.batch.get(h, k) for k in self.keep}
+                          if self.keep_ids is None or req.rid in self.keep_ids else {})
                 self.batch.remove(h)
                 results.append(Result(req.rid, req.n_fluid, req.steps, req.arrival, admitted, now,
                                       state, fields))
