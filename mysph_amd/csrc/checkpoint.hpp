@@ -39,6 +39,16 @@ size_t ckpt_bytes(const sphgpu_ctx* c) {
     return b;
 }
 
+// Bytes from a saved device address to the end of the saved allocation that
+// holds it (0 when none does).
+inline uint64_t bytes_after(const void* p, const std::vector<uint64_t>& old_base,
+                            const std::vector<uint64_t>& bytes) {
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    for (size_t k = 0; k < old_base.size(); ++k)
+        if (a >= old_base[k] && a < old_base[k] + bytes[k]) return old_base[k] + bytes[k] - a;
+    return 0;
+}
+
 // Map a device address of the saved context onto the restored allocations.
 template <typename T>
 bool remap(T*& p, const std::vector<uint64_t>& old_base, const std::vector<uint64_t>& bytes,
@@ -155,6 +165,32 @@ extern "C" int sphgpu_checkpoint_restore(const void* buf, int64_t bytes, sphgpu_
         need += pad8(rec[1]);
     }
     if ((int64_t)need > bytes) return create_fail(nullptr, SPHGPU_E_ARG, "truncated checkpoint");
+    // what the header and the saved config say must agree with each other and
+    // with the allocations before anything is created on the device: a
+    // corrupted blob would otherwise send kernels past the ends of the arrays
+    {
+        std::string why;
+        if (validate_config(&cfg, &why)) return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint config: " + why);
+        const bool counts_ok = h.capacity > 0 && h.n >= 1 && h.n <= h.capacity && h.n_real >= 0 &&
+                               h.n_dummy >= 0 && h.n_real + h.n_dummy == h.n && h.n <= INT32_MAX;
+        if (!counts_ok) return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: inconsistent particle counts");
+        if (h.dim != cfg.dim || (h.dim != 2 && h.dim != 3))
+            return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: dim does not match the config");
+        if (P.n != h.n || P.n_real != h.n_real || P.dim != h.dim)
+            return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: counts do not match the run's parameters");
+        if (h.nblk != (int32_t)((h.n + kBlock - 1) / kBlock) || h.step < 1)
+            return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: block count or step out of range");
+        // every per-particle array must hold n elements from where it starts
+        const struct { const void* p; uint64_t elem; } arrays[] = {
+            {D.posh, sizeof(double4)}, {D.vm, sizeof(double4)}, {D.rho, sizeof(double)},
+            {D.e, sizeof(double)}, {D.c, sizeof(double)}, {D.p, sizeof(double)},
+            {D.meta, sizeof(int4)}, {D.dv, sizeof(double4)}, {D.de, sizeof(double)},
+            {D.vprev, sizeof(double4)}, {D.th, sizeof(double4)}, {D.key, sizeof(uint32_t)},
+            {D.perm, sizeof(int)}, {D.cpos, sizeof(float4)}};
+        for (const auto& a : arrays)
+            if (!a.p || bytes_after(a.p, old_base, nbytes) < (uint64_t)h.n * a.elem)
+                return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint: a particle array is smaller than its count");
+    }
     sphgpu_ctx* c = nullptr;
     int rc = sphgpu_create(&cfg, h.capacity, &c);
     if (rc) return rc;

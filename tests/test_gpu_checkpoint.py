@@ -50,3 +50,38 @@ def test_restored_run_continues_bitwise(name):
             b.advance(37)
             for k in FIELDS:
                 assert np.array_equal(b.get(k), ref[k]), (name, k)
+
+
+def test_tampered_header_is_refused():
+    """A checkpoint whose header disagrees with itself or with its arrays is
+    refused with E_ARG before anything is created on the device (header:
+    magic u64, six u32, then int64 capacity @32, n @40, n_real @48, n_dummy
+    @56, ..., step @88, int32 dim @96, nblk @100)."""
+    from mysph_amd.sphgpu import SphGpuError
+    case, prec = _case("dam2d")
+    with Simulation(case) as a:
+        a.advance(3)
+        blob = a.checkpoint()
+    n = case["x"].shape[0]
+
+    def tamper(off, dtype, value):
+        b = blob.copy()
+        b[off:off + np.dtype(dtype).itemsize] = np.frombuffer(np.array([value], dtype=dtype).tobytes(),
+                                                               dtype=np.uint8)
+        return b
+
+    bad = {
+        "n above capacity": tamper(40, "<i8", n + 1),
+        "counts do not add up": tamper(48, "<i8", case["n_real"] + 5),
+        "negative dummies": tamper(56, "<i8", -1),
+        "dim": tamper(96, "<i4", 4),
+        "block count": tamper(100, "<i4", 1),
+        "step": tamper(88, "<i8", 0),
+        "truncated": blob[: blob.size // 2],
+    }
+    for what, b in bad.items():
+        with pytest.raises(SphGpuError) as e:
+            Simulation.restore(b)
+        assert e.value.code == -6, what
+    with Simulation.restore(blob) as c:     # the untouched blob still restores
+        assert c.n == n
