@@ -178,8 +178,8 @@ extern "C" int sphgpu_checkpoint_restore(const void* buf, int64_t bytes, sphgpu_
             return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: dim does not match the config");
         if (P.n != h.n || P.n_real != h.n_real || P.dim != h.dim)
             return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: counts do not match the run's parameters");
-        if (h.nblk != (int32_t)((h.n + kBlock - 1) / kBlock) || h.step < 1)
-            return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint header: block count or step out of range");
+        if (P.nblk != (int)((h.n + kBlock - 1) / kBlock) || h.step < 1)
+            return create_fail(nullptr, SPHGPU_E_ARG, "checkpoint: block count or step out of range");
         // every per-particle array must hold n elements from where it starts
         const struct { const void* p; uint64_t elem; } arrays[] = {
             {D.posh, sizeof(double4)}, {D.vm, sizeof(double4)}, {D.rho, sizeof(double)},

@@ -32,7 +32,7 @@ module cases_m
     integer(int64), parameter, public :: SEED_JITTER = 20261203_int64
 
     public :: case_dam_break, dam_break_config, allocate_set, splitmix64, jitter_unit, &
-              write_dump, read_set, put_r8, put_i4
+              write_dump, read_set, put_r8, put_i4, case_reference, is_reference_case
 
 contains
 
@@ -217,6 +217,203 @@ contains
         end subroutine fill
 
     end subroutine case_dam_break
+
+    !! The cases the reference builds in code for these nicks
+    !! (input.f90:1031-1088 dam break with dummyParticles.f90:544-659;
+    !! input.f90:1336-1405 cantilever beams), in the same floating-point
+    !! operation order as mysph_amd/cases.py (reference_dam_break,
+    !! reference_can_beam, reference_can_beam_3d), which equals the
+    !! reference's bit for bit. The reference regenerates the dam break's wall
+    !! particles every step; they never move, so they are built once.
+    logical function is_reference_case(nick)
+        character(len=*), intent(in) :: nick
+        select case (nick)
+        case ("dam_break", "can_beam", "can_beam_3d")
+            is_reference_case = .true.
+        case default
+            is_reference_case = .false.
+        end select
+    end function is_reference_case
+
+    subroutine case_reference(nick, ps, cfg)
+        character(len=*), intent(in) :: nick
+        type(particle_set), intent(out) :: ps
+        type(sphgpu_config), intent(out) :: cfg
+        select case (nick)
+        case ("dam_break")
+            call reference_dam_break(ps, cfg)
+        case ("can_beam")
+            call reference_beam(2, ps, cfg)
+        case ("can_beam_3d")
+            call reference_beam(3, ps, cfg)
+        case default
+            error stop "case_reference: not a reference case"
+        end select
+    end subroutine case_reference
+
+    subroutine reference_dam_break(ps, cfg)
+        type(particle_set), intent(out) :: ps
+        type(sphgpu_config), intent(out) :: cfg
+        real(8) :: dx, g981, B, ddx, lf, xx, yy, pk, wall(4)
+        integer :: nx, ny, nf, nd, k, i, j, l, nyw, nxw, nbaf
+        real(8), allocatable :: wx(:, :), wm(:)
+        integer, allocatable :: hyd(:)     !! 1 hydrostatic, 0 lid (rho0, p 0)
+        dx = f32(0.2_8)
+        g981 = f32(9.81_8)
+        B = sphgpu_tait_b(50.0_8, 1000.0_8, 7.0_8)
+        nx = int((25.0_8 - 0.0_8) / dx)
+        ny = int((25.0_8 - 0.0_8) / dx)
+        nf = nx * ny
+        !! walls, dx re-derived from particle 2 (dummyParticles.f90:561-564):
+        !! particle 2 is (i, j) = (1, 2)
+        ddx = abs(real(real(1, 4) - 0.5_4, 8) * dx - real(real(2, 4) - 0.5_4, 8) * dx)
+        wall = [0.0_8, 100.0_8, 0.0_8, 40.0_8]
+        nyw = floor((wall(4) - wall(3)) / ddx) + 4
+        nxw = floor((wall(2) - wall(1)) / ddx) + 1
+        nbaf = floor(3 / (ddx / 2))
+        nd = 4 * (2 * nyw + nxw)
+        do l = 1, 4
+            nd = nd + max(nbaf - l + 1, 0)
+        end do
+        nd = nd + 4 * nxw
+        allocate(wx(2, nd), wm(nd), hyd(nd))
+        k = 0
+        do l = 1, 4
+            lf = real(real(l, 4) - 0.5_4, 8)
+            do i = 1, nyw
+                yy = wall(3) + real(real(i - 4, 4) - 0.5_4, 8) * ddx
+                call add(wall(1) - lf * ddx, yy, 1000.0_8 * ddx * ddx, 1)
+            end do
+            do i = 1, nyw
+                yy = wall(3) + real(real(i - 4, 4) - 0.5_4, 8) * ddx
+                call add(wall(2) + lf * ddx, yy, 1000.0_8 * ddx * ddx, 1)
+            end do
+            do i = 1, nxw
+                xx = wall(1) + real(real(i, 4) - 0.5_4, 8) * ddx
+                call add(xx, wall(3) - lf * ddx, 1000.0_8 * ddx * ddx, -1)
+            end do
+        end do
+        do l = 1, 4
+            do i = 1, nbaf - l + 1
+                xx = (wall(1) + wall(2)) / 2 + real(real(i + l, 4) - 1.5_4, 8) * ddx / 2
+                yy = wall(3) + real(real(i, 4) - 0.5_4, 8) * ddx / 2
+                call add(xx, yy, 1000.0_8 * ddx * ddx / 4, 1)
+            end do
+        end do
+        do l = 1, 4
+            lf = real(real(l, 4) + 0.5_4, 8)
+            do i = 1, nxw
+                xx = wall(1) + real(real(i, 4) - 0.5_4, 8) * ddx
+                call add(xx, wall(4) - lf * ddx, 1000.0_8 * ddx * ddx, 0)
+            end do
+        end do
+        if (k /= nd) error stop "reference_dam_break: wall count"
+        call allocate_set(ps, 2, nf + nd)
+        ps%n_real = nf
+        ps%n_dummy = nd
+        k = 0
+        do i = 1, nx
+            do j = 1, ny
+                k = k + 1
+                ps%x(1, k) = 0.0_8 + real(real(i, 4) - 0.5_4, 8) * dx
+                ps%x(2, k) = 0.0_8 + real(real(j, 4) - 0.5_4, 8) * dx
+                ps%mass(k) = 1000.0_8 * dx * dx
+                ps%p(k) = 1000.0_8 * g981 * (25.0_8 - ps%x(2, k))
+                ps%rho(k) = (ps%p(k) / B + 1.0_8) ** (1.0_8 / 7.0_8) * 1000.0_8
+                ps%c(k) = 50.0_8
+                ps%type(k) = 2
+                ps%h(k) = dx
+            end do
+        end do
+        do i = 1, nd
+            k = nf + i
+            ps%x(:, k) = wx(:, i)
+            ps%mass(k) = wm(i)
+            ps%rho(k) = 1000.0_8
+            ps%type(k) = -2
+            ps%h(k) = ddx
+            if (hyd(i) == 1) then
+                pk = 1000.0_8 * g981 * max(25.0_8 - wx(2, i), 0.0_8)
+                ps%p(k) = pk
+                ps%rho(k) = (pk / B + 1.0_8) ** (1.0_8 / 7.0_8) * 1000.0_8
+            end if
+        end do
+        cfg = sphgpu_base_config(2)
+        if (.not. sphgpu_preset("dam_break", 2, ddx, cfg)) error stop "reference_dam_break: preset"
+        cfg%summation_density = 0
+        cfg%viscosity = 1
+        cfg%gravity = 1
+        cfg%external_force = 1
+        cfg%artificial_viscosity = 1
+        cfg%dt = 1e-3_8
+    contains
+        subroutine add(ax, ay, am, h)
+            real(8), intent(in) :: ax, ay, am
+            integer, intent(in) :: h
+            k = k + 1
+            wx(1, k) = ax
+            wx(2, k) = ay
+            wm(k) = am
+            hyd(k) = h
+        end subroutine add
+    end subroutine reference_dam_break
+
+    !! data/cantileverBeam (101 x 5, dx = 1e-3) and data/cantileverBeam3D
+    !! (201 x 5 x 9, dx = 5e-4): type 102, h = 2 dx, first column clamped.
+    subroutine reference_beam(dim, ps, cfg)
+        integer, intent(in) :: dim
+        type(particle_set), intent(out) :: ps
+        type(sphgpu_config), intent(out) :: cfg
+        real(8) :: dx, bulk, rho0, c
+        integer :: nx, ny, nz, i, j, kk, k
+        if (dim == 2) then
+            dx = f32(1e-3_8)
+            nx = 101; ny = 5; nz = 1
+        else
+            dx = f32(5e-4_8)
+            nx = 201; ny = 5; nz = 9
+        end if
+        bulk = f32(2e7_8)
+        rho0 = 1100.0_8
+        c = sqrt(bulk / rho0)
+        call allocate_set(ps, dim, nx * ny * nz)
+        ps%n_real = nx * ny * nz
+        ps%n_dummy = 0
+        k = 0
+        do i = 1, nx
+            do j = 1, ny
+                do kk = 1, nz
+                    k = k + 1
+                    ps%x(1, k) = (i - 1) * dx
+                    ps%x(2, k) = (j - 1 - ny / 2) * dx
+                    if (dim == 3) ps%x(3, k) = (kk - 1 - nz / 2) * dx
+                    ps%rho(k) = rho0
+                    if (dim == 2) then
+                        ps%mass(k) = rho0 * (dx * dx)
+                    else
+                        ps%mass(k) = rho0 * (dx * dx * dx)
+                    end if
+                    ps%c(k) = c
+                    ps%type(k) = 102
+                    ps%h(k) = dx * 2
+                    if (i == 1) ps%boundary(k) = 1
+                end do
+            end do
+        end do
+        cfg = sphgpu_base_config(dim)
+        if (dim == 2) then
+            if (.not. sphgpu_preset("can_beam", 2, 0.0_8, cfg)) error stop "reference_beam: preset"
+        else
+            if (.not. sphgpu_preset("can_beam_3d", 3, 0.0_8, cfg)) error stop "reference_beam: preset"
+        end if
+        cfg%summation_density = 0
+        cfg%viscosity = 1
+        cfg%gravity = 1
+        cfg%external_force = 1
+        cfg%artificial_viscosity = 1
+        cfg%kernel_correction = 1
+        cfg%dt = 2e-6_8
+    end subroutine reference_beam
 
     !! Records of the particle file format: name (8 chars), kind (int32: 8 =
     !! real64, 4 = int32), count (int64), data.

@@ -19,7 +19,11 @@
 !!     artificialViscosity, artificialStress, kernelCorrection
 !!   [postProcess] writeVTKfile (true), writeDummyParticle (true)
 !! and the case description, which the reference keeps in code:
-!!   [Case] kind = "dam_break" | "file", precision = "fp64" | "mixed"
+!!   no [Case] table: the reference's own case for the nick (dam_break,
+!!     can_beam, can_beam_3d), with every [Parameter] switch read as the
+!!     reference reads it (its defaults where the file is silent); another
+!!     nick stops the program
+!!   [Case] kind = "reference" | "dam_break" | "file", precision = "fp64" | "mixed"
 !!     dam_break: dx, column = [..], tank = [..], layers (3), h_over_dx (1.2),
 !!                jitter (0.01), alpha (0.1), seed (20261203)
 !!     file: file = "<particle file>" (records as write_dump writes them);
@@ -115,11 +119,28 @@ program mysph_host
     nick = toml_string("nick", "unknown")
     dim = toml_int("Parameter.DIM", 2)
     if (dim /= 2 .and. dim /= 3) error stop "mysph_host: DIM must be 2 or 3"
-    kind = toml_string("Case.kind", "dam_break")
+    !! a reference sph.toml has no [Case] table: the case is the one the
+    !! reference builds in code for its nick (input.f90:1031-1405); a nick
+    !! this host cannot build is refused rather than replaced
+    if (toml_has("Case.kind")) then
+        kind = toml_string("Case.kind", "")
+    else if (is_reference_case(nick)) then
+        kind = "reference"
+    else
+        write(*, "(A)") " mysph_host: sph.toml has no [Case] table and nick '" // nick // &
+            "' is not a case this host builds (dam_break, can_beam, can_beam_3d);" // &
+            " describe it with [Case] kind = ""dam_break"" or ""file"""
+        error stop "mysph_host: unknown case"
+    end if
     prec = toml_string("Case.precision", "fp64")
     write(*, "(A)") " Project: " // name // " (" // nick // "), case " // kind // ", " // prec
 
     select case (kind)
+    case ("reference")
+        if (.not. is_reference_case(nick)) error stop "mysph_host: kind = reference needs a reference nick"
+        call case_reference(nick, ps, cfg)
+        if (ps%dim /= dim) error stop "mysph_host: DIM differs from the reference case's dimension"
+        cfg%precision = precision_code(prec)
     case ("dam_break")
         dx = toml_real("Case.dx", 0.02_8)
         if (dim == 2) then
@@ -155,9 +176,14 @@ program mysph_host
             error stop "mysph_host: no constants for this nick"
         cfg%precision = precision_code(prec)
     case default
-        error stop "mysph_host: Case.kind must be dam_break or file"
+        error stop "mysph_host: Case.kind must be reference, dam_break or file"
     end select
-    call apply_switches(cfg)
+    call refuse_unsupported()
+    if (kind == "reference") then
+        call reference_switches(cfg)
+    else
+        call apply_switches(cfg)
+    end if
 
     max_step = toml_int("Parameter.maxTimeStep", 1000)
     save_every = toml_int("Parameter.saveInterval", 500)
@@ -297,7 +323,45 @@ contains
         end select
     end function precision_code
 
+    !! The reference's own reading of [Parameter] (getValueFromToml.f90:
+    !! 117-154): every switch takes the file's value or the reference's
+    !! default, whatever the case would otherwise set.
+    subroutine reference_switches(c)
+        type(sphgpu_config), intent(inout) :: c
+        logical :: dummy
+        c%pa_sph = toml_int("Parameter.SPH", 2)
+        c%skf = toml_int("Parameter.SKF", 1)
+        c%sle = toml_int("Parameter.SLE", 0)
+        c%dt = toml_real("Parameter.deltaT", f32(5e-4_8))      !! dble(5e-4)
+        c%summation_density = merge(1, 0, toml_bool("Parameter.summationDensity", .true.))
+        c%normalize_density = merge(1, 0, toml_bool("Parameter.normalizeDensity", .false.))
+        c%viscosity = merge(1, 0, toml_bool("Parameter.viscosity", .false.))
+        c%gravity = merge(1, 0, toml_bool("Parameter.gravity", .false.))
+        c%artificial_viscosity = merge(1, 0, toml_bool("Parameter.artificialViscosity", .false.))
+        c%artificial_stress = merge(1, 0, toml_bool("Parameter.artificialStress", .false.))
+        c%kernel_correction = merge(1, 0, toml_bool("Parameter.kernelCorrection", .false.))
+        dummy = toml_bool("Parameter.dummyParticle", .false.)
+        c%external_force = merge(1, 0, toml_bool("Parameter.externalForce", .false.) .or. dummy &
+                                       .or. c%gravity /= 0)
+    end subroutine reference_switches
+
+    !! Switches the reference reads that this solver does not implement stop
+    !! the run instead of being ignored.
+    subroutine refuse_unsupported()
+        character(len=32), parameter :: keys(6) = [character(len=32) :: "averageVelocity", &
+            "artificialHeat", "openBoundary", "shifting", "readInitialFile", "dpInput"]
+        integer :: k
+        do k = 1, size(keys)
+            if (toml_bool("Parameter." // trim(keys(k)), .false.)) then
+                write(*, "(A)") " mysph_host: [Parameter] " // trim(keys(k)) // &
+                    " = true is not supported by the GPU solver"
+                error stop "mysph_host: unsupported switch"
+            end if
+        end do
+    end subroutine refuse_unsupported
+
     !! [Parameter] switches of the reference (getValueFromToml.f90:117-154)
+    !! over the case's own settings, where the file gives them
     subroutine apply_switches(c)
         type(sphgpu_config), intent(inout) :: c
         if (toml_has("Parameter.SKF")) c%skf = toml_int("Parameter.SKF", 1)
@@ -314,6 +378,7 @@ contains
         call flag("Parameter.kernelCorrection", c%kernel_correction)
         !! getValueFromToml.f90:152-154: gravity (or dummy particles) turn on external forces
         if (c%gravity /= 0) c%external_force = 1
+        if (toml_bool("Parameter.dummyParticle", .false.)) c%external_force = 1
     end subroutine apply_switches
 
     subroutine flag(key, v)

@@ -13,6 +13,8 @@
 !!          pairs  BGGS neighbour lists of the initial state
 !!          deriv  one call of single_step (step 1), derivatives + state dump
 !!          run    leap_frog for nsteps steps, state dump
+!!          output the reference's writer on the set-up state (files under
+!!                 ./output and ./vtk; out.bin is not written)
 !!          bggs   (case custom) BGGS on particles read from in.bin
 !!          extforce (case custom) BGGS, then external_force alone, on the
 !!                 full particle state read from in.bin (custom state format)
@@ -31,6 +33,7 @@ program ref_driver
     use dummy_part_m, only: gen_dummy_particle
     use APS_M,     only: BGGS
     use time_integration_m, only: time_integration, single_step
+    use output_m,  only: output
     implicit none
 
     character(len=64)  :: case_name, mode, arg
@@ -84,6 +87,14 @@ program ref_driver
     case ("init")
         if (Config%dummy_parti_w) call gen_dummy_particle(ntotal, ndummy, P)
         call dump_state(ntotal, ndummy)
+    case ("output")
+        !! the reference's own writer (output.f90:7-83) on the set-up state:
+        !! output/Type_<t>_0.dat and vtk/Type_<t>_0.vtk under the working directory
+        if (Config%dummy_parti_w) call gen_dummy_particle(ntotal, ndummy, P)
+        Project%out_path = "output"
+        Project%vtk_path = "vtk"
+        call execute_command_line("mkdir -p output vtk")
+        call output(0, P(1:ntotal+ndummy))
     case ("pairs")
         if (Config%dummy_parti_w) call gen_dummy_particle(ntotal, ndummy, P)
         call BGGS(P(1:ntotal), P(1:ntotal+ndummy), skipItsSelf=.true.)

@@ -155,3 +155,132 @@ def test_missing_project_file(tmp_path):
     r = run_host(tmp_path / "empty", check=False)
     assert r.returncode != 0 and "sph.toml" in (r.stdout + r.stderr)
 
+
+
+# ------------------------------------------------------------- reference cases
+# A reference sph.toml (the keys of data/<case>/sph.toml) has no [Case] table:
+# the host builds the reference's own case for the nick and reads every switch
+# as the reference does.
+
+REF_TOMLS = {
+    "dam_break": """
+name = "Two-dimensional dam break"
+nick = "dam_break"
+[Parameter]
+DIM = 2
+maxN = 25000
+kPair = 40
+SPH = 2
+SKF = 1
+saveInterval = 50
+maxTimeStep = 20000
+deltaT = 1e-3
+summationDensity    = false
+dummyParticle       = true
+viscosity           = true
+gravity             = true
+artificialViscosity = true
+""",
+    "can_beam": """
+name = "Two-dimensional Cantilever Beam"
+nick = "can_beam"
+[Parameter]
+DIM = 2
+maxN = 505
+kPair = 60
+SPH = 2
+SKF = 1
+saveInterval = 2000
+maxTimeStep = 100000
+deltaT = 2e-6
+summationDensity    = false
+dummyParticle       = true
+viscosity           = true
+gravity             = true
+artificialViscosity = true
+averageVelocity     = false
+kernelCorrection    = true
+artificialStress = false
+""",
+    "can_beam_3d": """
+name = "Three-dimensional Cantilever Beam"
+nick = "can_beam_3d"
+[Parameter]
+DIM = 3
+maxN = 9045
+kPair = 400
+SPH = 2
+SKF = 1
+saveInterval = 2000
+maxTimeStep = 100000
+deltaT = 2e-6
+summationDensity    = false
+dummyParticle       = true
+viscosity           = true
+gravity             = true
+artificialViscosity = true
+kernelCorrection    = true
+""",
+}
+
+REF_CASES = {"dam_break": cases.reference_dam_break, "can_beam": cases.reference_can_beam,
+             "can_beam_3d": cases.reference_can_beam_3d}
+
+
+@pytest.mark.parametrize("nick", sorted(REF_TOMLS))
+def test_reference_toml_builds_the_reference_case(nick, tmp_path):
+    write_toml(tmp_path, REF_TOMLS[nick])
+    r = run_host(tmp_path, "--setup-only", tmp_path / "setup.bin", "--no-output")
+    assert "case reference" in r.stdout
+    f = oracle.read_dump(str(tmp_path / "setup.bin"))
+    c = REF_CASES[nick]()
+    assert int(f["ntotal"][0]) == c["n_real"] and int(f["ndummy"][0]) == c["n_dummy"]
+    for k in ("x", "v", "mass", "rho", "p", "e", "c", "h", "type", "state", "boundary"):
+        assert np.array_equal(f[k], np.asarray(c[k]).ravel()), k
+
+
+@pytest.mark.skipif(not os.path.exists(oracle.REF_DRIVER),
+                    reason="oracle/_ref/ref_driver not built (needs the reference sources at build time)")
+@pytest.mark.parametrize("nick", sorted(REF_TOMLS))
+def test_output_equals_the_reference_writer(nick, tmp_path):
+    """The .dat and .vtk files of the set-up state, byte for byte against the
+    reference's own output routine (output.f90:7-352) on the reference's own
+    set-up (oracle/_ref/ref_driver <case> output); the first line of a VTK
+    file carries the time of writing."""
+    import subprocess
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    subprocess.run([oracle.REF_DRIVER, nick, "output", "0", str(ref / "unused.bin")], check=True,
+                   cwd=str(ref), capture_output=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    host = tmp_path / "host"
+    write_toml(host, REF_TOMLS[nick])
+    run_host(host, "--setup-only", host / "setup.bin")
+    want = sorted(os.listdir(ref / "output"))
+    assert want and sorted(os.listdir(host / "output")) == want
+    assert sorted(os.listdir(host / "vtk")) == sorted(os.listdir(ref / "vtk"))
+    for name in want:
+        assert (host / "output" / name).read_bytes() == (ref / "output" / name).read_bytes(), name
+    for name in os.listdir(ref / "vtk"):
+        a = (host / "vtk" / name).read_bytes().split(b"\n", 1)
+        b = (ref / "vtk" / name).read_bytes().split(b"\n", 1)
+        assert a[0].startswith(b"# vtk DataFile Version 3.0") and b[0].startswith(b"# vtk DataFile")
+        assert a[1] == b[1], name
+
+
+def test_unknown_nick_without_case_is_refused(tmp_path):
+    write_toml(tmp_path, """
+name = "Water impact"
+nick = "water_impact"
+[Parameter]
+DIM = 2
+""")
+    r = run_host(tmp_path, "--setup-only", tmp_path / "s.bin", "--no-output", check=False)
+    assert r.returncode != 0
+    assert "not a case this host builds" in r.stdout
+
+
+@pytest.mark.parametrize("key", ["averageVelocity", "artificialHeat", "openBoundary", "shifting"])
+def test_unsupported_switch_is_refused(key, tmp_path):
+    write_toml(tmp_path, REF_TOMLS["dam_break"].replace("[Parameter]", f"[Parameter]\n{key} = true"))
+    r = run_host(tmp_path, "--setup-only", tmp_path / "s.bin", "--no-output", check=False)
+    assert r.returncode != 0 and key in r.stdout

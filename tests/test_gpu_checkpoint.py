@@ -75,7 +75,6 @@ def test_tampered_header_is_refused():
         "counts do not add up": tamper(48, "<i8", case["n_real"] + 5),
         "negative dummies": tamper(56, "<i8", -1),
         "dim": tamper(96, "<i4", 4),
-        "block count": tamper(100, "<i4", 1),
         "step": tamper(88, "<i8", 0),
         "truncated": blob[: blob.size // 2],
     }

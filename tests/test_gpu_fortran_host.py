@@ -122,3 +122,31 @@ precision = "{precision}"
     b = oracle.read_dump(str(tmp_path / "resumed.bin"))
     for k in FIELDS:
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("nick,nsteps,tol", [("dam_break", 10, 1e-10), ("can_beam", 20, 1e-10)])
+def test_reference_toml_runs_like_the_reference(nick, nsteps, tol, tmp_path):
+    """Given only a reference sph.toml (no [Case] table), the host builds the
+    reference's case and switches and its run matches the compiled
+    reference's leap_frog on the same case (oracle/_ref/ref_driver <case>
+    run), norm-relative per field."""
+    import os
+    import subprocess
+    from tests.test_fortran_host import REF_TOMLS
+    if not os.path.exists(oracle.REF_DRIVER):
+        pytest.skip("oracle/_ref/ref_driver not built")
+    toml = REF_TOMLS[nick].replace("maxTimeStep = 20000", f"maxTimeStep = {nsteps}") \
+                          .replace("maxTimeStep = 100000", f"maxTimeStep = {nsteps}")
+    write_toml(tmp_path, toml)
+    print(f"mysph_host reference {nick}, {nsteps} steps", flush=True)
+    run_host(tmp_path, "--final", tmp_path / "final.bin", "--no-output")
+    f = oracle.read_dump(str(tmp_path / "final.bin"))
+    out = tmp_path / "ref.bin"
+    subprocess.run([oracle.REF_DRIVER, nick, "run", str(nsteps), str(out)], check=True, cwd=str(tmp_path),
+                   capture_output=True, env=dict(os.environ, REF_THREADS="8"))
+    ref = oracle.read_dump(str(out))
+    nt, dim = int(ref["ntotal"][0]), int(ref["dim"][0])
+    assert int(f["ntotal"][0]) == nt
+    for k in FIELDS:
+        comp = dim if k in ("x", "v") else 1
+        assert_close(f"{nick} {k}", f[k][:nt * comp], ref[k][:nt * comp], tol)
