@@ -93,7 +93,9 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
     const int cap = count_only ? 0 : P.verlet_cap;
     int cnt = 0;
     if (active) {
-        const double cutv = qi.w * (double)P.scale_k + P.skin;
+        // an evolving h (general path) may grow up to the planned limit
+        // before the next build
+        const double cutv = (P.sle ? P.hmax : qi.w) * (double)P.scale_k + P.skin;
         const RefGrid g = *P.grid;
         const double u = 5.9604644775390625e-8;
         const double e = g.pos_err + u * cutv;

@@ -1,0 +1,161 @@
+// Candidate list for the general path of libsphgpu (included by sphgpu.hip
+// after the fluid fast path, whose list build it shares): the solid, KGC,
+// summation-density, SLE and SPH 1/3 physics keep their pair kernels over an
+// exact neighbour list, but that list is no longer found by a cell sweep every
+// step. The cell sort and the candidate list (every j within cut + skin) are
+// rebuilt only when some particle has moved skin/2 since the last build, and
+// each step filters the candidates with the reference's exact rule into the
+// exact list.
+//
+// Exactness is the fast path's argument: a pair within the cutoff at this
+// step was within cutoff + skin at the build, so it is a candidate, and the
+// filter accepts exactly the reference's pairs (asymmetricSearch.f90:209-237).
+// With an evolving h the list is built for the planned largest h, which the
+// smoothing-length update enforces (k_hsml).
+
+// Predict + EOS (as k_point_build), the FP32 position record of the filter's
+// pre-test, the reference cells of this step when the reference window can
+// exclude a pair, and the displacement vote for a rebuild. mode bit 0: first
+// step (no predict); bit 1: no predict at all; bit 2: sizing pass (the state
+// is left as uploaded).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_point_gen(Params P, Dev D, int mode) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    bool moved = false;
+    if (s < P.n) {
+        const int4 mt = D.meta[s];
+        const bool real = mt.w < P.n_real;
+        if (real && !(mode & 2)) predict_one<DIM>(P, D, s, mt, mode & 1);
+        if (!(mode & 4)) {
+            if (P.pa_sph == 3) D.pold[s] = D.p[s];
+            if (!P.summation_density) eos_th(P, D, s, mt);
+        }
+        const double4 q = D.posh[s];
+        D.posf[s] = cell_offset_record(P, q);
+        if (mt.y != -1 && P.grid->need_win) {
+            const RefGrid g = *P.grid;
+            bool bad = false;
+            const double xs[3] = {q.x, q.y, q.z};
+            int rc[3] = {1, 1, 1};
+            for (int d = 0; d < DIM; ++d) rc[d] = ref_locate(xs[d], g.minc[d], g.maxc[d], g.num[d], &bad);
+            if (bad) atomicOr(P.err, (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) ? kErrOutOfRange
+                                                                                      : kErrNonFinite);
+            D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
+        }
+        moved = moved_half_skin<false>(P, D, s, q);
+    }
+    // every lane reaches the vote (one atomic per wave)
+    vote_rebuild(P, moved);
+}
+
+// Exact neighbour list of this step from the candidate list: the reference's
+// distance test and window on every candidate, kept in candidate order (the
+// order of the cell sweep at the build), nbr[k * ldn + s] as the pair kernels
+// read it. The distance is tested in FP32 from the cell-offset records first;
+// only a candidate within the FP32 error bound of the cutoff (or off the grid)
+// loads its FP64 position for the reference's test. A particle whose
+// candidates outgrew its list walks its cell window of the last sort instead.
+// Closes the rebuild gate of the step: every gated kernel ran before this
+// launch.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.vflag[0] = 0ull;
+        P.vflag[1] = 0ull;
+    }
+    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    if (mt.y == -1) {
+        D.ncnt[s] = 0;
+        return;
+    }
+    const double4 qi = D.posh[s];
+    const double cut = qi.w * (double)P.scale_k;
+    const double cut2 = cut * cut;
+    const double lo2 = cut2 * (1.0 - 1e-12), hi2 = cut2 * (1.0 + 1e-12);
+    // FP32 r^2 from cell offsets is within M of the FP64 one
+    const float4 pfi = D.posf[s];
+    const uint32_t pk_i = __float_as_uint(pfi.w);
+    int cxi, cyi, czi;
+    unpack_cell<DIM>(pk_i, cxi, cyi, czi);
+    const float csf = (float)P.cs;
+    float lof, hif;
+    {
+        const double u = 5.9604644775390625e-8;
+        const double e = 2.0 * u * P.cs + u * cut;
+        const double M = 2.0 * 1.7320508075688772 * cut * e + 3.0 * e * e + 3.0 * u * cut2;
+        lof = (float)(cut2 - 2.0 * M - 1e-6 * cut2);
+        hif = (float)(cut2 + 2.0 * M + 1e-6 * cut2);
+    }
+    // reference window (only when it can exclude a pair of this particle)
+    const RefGrid g = *P.grid;
+    int wr[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
+    bool need_win = false;
+    const uint32_t rci = g.need_win ? D.refcell[s] : 0u;
+    for (int d = 0; d < DIM && g.need_win; ++d) {
+        const double span = g.maxc[d] - g.minc[d];
+        wr[d] = (int)(qi.w / span * g.num[d]) + 1;   // cellNumPerHSML, asymmetricSearch.f90:209-211
+        rc[d] = ref_axis(rci, d);
+        const double reach = cut / (span / g.num[d]) * (1.0 + 1e-12) + 1e-9;
+        if (wr[d] < (int)ceil(reach)) need_win = true;
+    }
+    const int cap = P.nbr_cap;
+    int cnt = 0;
+    auto take = [&](int j, const float4& pf) {
+        const uint32_t pk = __float_as_uint(pf.w);
+        int cxj, cyj, czj;
+        unpack_cell<DIM>(pk, cxj, cyj, czj);
+        const float dx = (pfi.x - pf.x) + (float)(cxi - cxj) * csf;
+        const float dy = (pfi.y - pf.y) + (float)(cyi - cyj) * csf;
+        const float dz = DIM == 3 ? (pfi.z - pf.z) + (float)(czi - czj) * csf : 0.0f;
+        const float r2 = (int)(pk | pk_i) < 0 ? lof : dx * dx + dy * dy + dz * dz;
+        if (!(r2 < hif)) return;
+        if (!(r2 < lof) && exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) return;
+        if (need_win) {
+            const uint32_t rcj = D.refcell[j];
+            bool inwin = true;
+            for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
+            if (!inwin) return;
+        }
+        if (cnt < cap) D.nbr[(int64_t)cnt * P.ldn + s] = j;
+        ++cnt;
+    };
+    const int nc = D.vcnt[s];
+    if (nc > P.verlet_cap) {
+        const uint32_t key = D.key[s];
+        const int cx = (int)(key % (uint32_t)P.nc[0]);
+        const int rest = (int)(key / (uint32_t)P.nc[0]);
+        const int cy = rest % P.nc[1];
+        const int cz = rest / P.nc[1];
+        const int W = P.win;
+        const int zlo = DIM == 3 ? max(cz - W, 0) : 0, zhi = DIM == 3 ? min(cz + W, P.nc[2] - 1) : 0;
+        const int ylo = max(cy - W, 0), yhi = min(cy + W, P.nc[1] - 1);
+        const int xlo = max(cx - W, 0), xhi = min(cx + W, P.nc[0] - 1);
+        for (int z = zlo; z <= zhi; ++z)
+            for (int y = ylo; y <= yhi; ++y) {
+                const int64_t row = (int64_t)P.nc[0] * ((int64_t)y + (int64_t)P.nc[1] * z);
+                const int ke = D.start[row + xhi + 1];
+                for (int k = D.start[row + xlo]; k < ke; ++k) {
+                    const int j = __float_as_int(D.cpos[k].w);
+                    if (j != s) take(j, D.posf[j]);
+                }
+            }
+    } else {
+        // the record of candidate k + 1 is in flight while candidate k is tested
+        const int* __restrict__ nl = D.vnbr + vlist_at(s, 0, P.verlet_cap);
+        int j = nc > 0 ? __builtin_nontemporal_load(nl) : s;
+        float4 pj = D.posf[(uint32_t)j];
+        for (int k = 0; k < nc; ++k) {
+            const int jn = k + 1 < nc ? __builtin_nontemporal_load(nl + (int64_t)(k + 1) * kVlistLanes) : j;
+            const float4 pn = D.posf[(uint32_t)jn];
+            take(j, pj);
+            j = jn;
+            pj = pn;
+        }
+    }
+    D.ncnt[s] = cnt;
+    if (cnt > cap) atomicOr(P.err, kErrPairCap);
+    if (cnt > __atomic_load_n(P.maxcnt, __ATOMIC_RELAXED)) atomicMax(P.maxcnt, cnt);
+}

@@ -113,6 +113,7 @@ struct Dev {
     int* vcnt;
     int* ecnt;        // accepted neighbours per particle in the last force pass
     unsigned long long* vflag;   // rebuild gate (see Gate)
+    int* sidx;        // general path: the slots of the real solid targets, in slot order
 };
 
 // Block min/max of positions -> partial[blockIdx*6 + (0..2 min, 3..5 max)].
@@ -579,6 +580,8 @@ __device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int
     D.ef[s] = e;
 }
 
+__device__ __forceinline__ float4 cell_offset_record(const Params& P, const double4& q);
+
 // FP32 EOS on the density deviation: the Tait term (rho/rho0)^gamma - 1 is
 // evaluated as a polynomial in u = drho/rho0 (gamma = 7) or expm1(gamma
 // log1p(u)), so it does not cancel.
@@ -611,8 +614,14 @@ __device__ __forceinline__ void eos_mixed_vals(const Params& P, const Dev& D, in
     rec.vm = make_float4((float)v.x, (float)v.y, (float)v.z, mass * irho);
     rec.th = make_float4(rho, p * irho * irho, (float)D.c[s], (float)q.w);
     D.prf[s] = rec;
-    // offset from the particle's own cell corner, exact cell size cs: dx of a
-    // pair is (off_i - off_j) + (c_i - c_j) cs, accurate to ~1e-7 of a cell
+    D.posf[s] = cell_offset_record(P, q);
+}
+
+// FP32 position record: the offset from the particle's own cell corner, with
+// the exact cell size cs: dx of a pair is (off_i - off_j) + (c_i - c_j) cs,
+// accurate to ~1e-7 of a cell. w: the packed cell, bit 31 set when the
+// particle is outside the grid (its offset is then not bounded by a cell).
+__device__ __forceinline__ float4 cell_offset_record(const Params& P, const double4& q) {
     double xs[3] = {q.x, q.y, q.z};
     int c[3] = {0, 0, 0};
     float off[3] = {0.0f, 0.0f, 0.0f};
@@ -630,7 +639,7 @@ __device__ __forceinline__ void eos_mixed_vals(const Params& P, const Dev& D, in
     uint32_t pk = P.dim == 3 ? ((uint32_t)c[0] | ((uint32_t)c[1] << 11) | ((uint32_t)c[2] << 22))
                              : ((uint32_t)c[0] | ((uint32_t)c[1] << 16));
     if (escaped) pk |= 0x80000000u;
-    D.posf[s] = make_float4(off[0], off[1], off[2], __uint_as_float(pk));
+    return make_float4(off[0], off[1], off[2], __uint_as_float(pk));
 }
 
 __device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, int type) {
@@ -765,6 +774,41 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
     if (cnt > __atomic_load_n(D.maxcnt, __ATOMIC_RELAXED)) atomicMax(D.maxcnt, cnt);
 }
 
+// Lanes per target of the general pair kernels: 1 (one lane walks a target's
+// whole neighbour list), or kLanesPerTarget for runs too small to fill the
+// chip with one lane per target and for the few solid targets of a fluid run,
+// whose lanes would otherwise walk long lists one dependent gather after
+// another. The lanes of a target take every kLanesPerTarget-th neighbour,
+// and a fixed shuffle tree adds their partial sums (the same order in every
+// run).
+constexpr int kLanesPerTarget = 8;
+// real particles up to which a run takes kLanesPerTarget lanes per target
+constexpr int kSmallRun = 1 << 18;
+
+// Slot of this lane's target (P.n: none) and the lane's place in its group.
+// LPT == 1: one lane per slot, XCD-aware block order. LPT > 1: groups of LPT
+// lanes over the listed slots (list) or over all slots; a group is whole, so
+// every lane of a group takes the same early exits.
+template <int LPT>
+__device__ __forceinline__ int target_slot(const Params& P, const int* list, int count, int& sub) {
+    if (LPT == 1) {
+        sub = 0;
+        return xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    }
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    sub = g % LPT;
+    const int t = g / LPT;
+    if (list) return t < count ? list[t] : P.n;
+    return t;
+}
+
+// Sum of v over the LPT lanes of a group, in a fixed tree order.
+template <int LPT, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    for (int o = LPT / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // Pair geometry of target i and neighbour j: dx = x_i - x_j, r, W, grad W.
 // R = double, or float in the mixed general path (dx formed in FP64 and
 // rounded once; the corrected-gradient solve stays FP64).
@@ -795,9 +839,10 @@ __device__ __forceinline__ void pair_geo(const Params& P, const double4& qi, con
 }
 
 // kernelGradientCorrection.f90:20-35: A_i = sum_j V_j (x_j - x_i)(e) dW(d).
-template <int DIM, typename R>
+template <int DIM, typename R, int LPT>
 __global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
-    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    int sub;
+    const int s = target_slot<LPT>(P, nullptr, 0, sub);
     if (s >= P.n) return;
     int4 mt = D.meta[s];
     if (mt.w >= P.n_real || mt.y == -1) return;
@@ -808,7 +853,7 @@ __global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
         R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
         int n = min(D.ncnt[s], P.nbr_cap);
         int ai = mt.x < 0 ? -mt.x : mt.x;
-        for (int k = 0; k < n; ++k) {
+        for (int k = sub; k < n; k += LPT) {
             int j = D.nbr[(int64_t)k * P.ldn + s];
             int tj = D.meta[j].x;
             int aj = tj < 0 ? -tj : tj;
@@ -821,13 +866,18 @@ __global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
                 for (int e = 0; e < DIM; ++e) A[d * DIM + e] += (-g.dx[e]) * g.gw[d] * V;
         }
     }
+    if (LPT > 1) {
+        for (int q = 0; q < DIM * DIM; ++q) A[q] = group_sum<LPT>(A[q]);
+        if (sub) return;
+    }
     for (int q = 0; q < DIM * DIM; ++q) D.kgcA[(int64_t)s * DIM * DIM + q] = (double)A[q];
 }
 
 // density.f90:10-80 (summation density, optional Shepard normalisation).
-template <int DIM, typename R>
+template <int DIM, typename R, int LPT>
 __global__ __launch_bounds__(kBlock) void k_sumdens(Params P, Dev D) {
-    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    int sub;
+    const int s = target_slot<LPT>(P, nullptr, 0, sub);
     if (s >= P.n) return;
     int4 mt = D.meta[s];
     if (mt.w >= P.n_real) return;
@@ -835,12 +885,13 @@ __global__ __launch_bounds__(kBlock) void k_sumdens(Params P, Dev D) {
     R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
     R w0 = kernel_w0<DIM, R>(P.skf, (R)qi.w);
     R mi = (R)D.vm[s].w;
-    R rho = mi * w0;
+    // the self terms go to the first lane of a target's group
+    R rho = sub == 0 ? mi * w0 : (R)0.0;
     R wi = (R)0.0;
     bool norm = P.normalize_density && D.div[s].x < 1.5;
-    if (norm) wi = mi / (R)D.rho[s] * w0;
+    if (norm && sub == 0) wi = mi / (R)D.rho[s] * w0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
-    for (int k = 0; k < n; ++k) {
+    for (int k = sub; k < n; k += LPT) {
         int j = D.nbr[(int64_t)k * P.ldn + s];
         double4 qj = D.posh[j];
         PairGeo<DIM, R> g;
@@ -848,6 +899,11 @@ __global__ __launch_bounds__(kBlock) void k_sumdens(Params P, Dev D) {
         R mj = (R)D.vm[j].w;
         rho += mj * g.w;
         if (norm) wi += mj / (R)D.rho[j] * g.w;
+    }
+    if (LPT > 1) {
+        rho = group_sum<LPT>(rho);
+        wi = group_sum<LPT>(wi);
+        if (sub) return;
     }
     if (norm) rho = rho / wi;
     D.rho_new[s] = (double)rho;
@@ -863,9 +919,10 @@ __global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
 
 // Strain-rate (and solid rotation-rate) sums, internalForce.f90:48-100, and the
 // fluid viscous heating tdsdt (:108-112). T(d,e) = sum_j V_j (v_j - v_i)_d dW_e.
-template <int DIM, typename R>
+template <int DIM, typename R, int LPT>
 __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
-    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    int sub;
+    const int s = target_slot<LPT>(P, nullptr, 0, sub);
     if (s >= P.n) return;
     int4 mt = D.meta[s];
     const Material* m = find_mat(P, mt.x);
@@ -880,7 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
         R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
         const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
         int n = min(D.ncnt[s], P.nbr_cap);
-        for (int k = 0; k < n; ++k) {
+        for (int k = sub; k < n; k += LPT) {
             int j = D.nbr[(int64_t)k * P.ldn + s];
             double4 qj = D.posh[j];
             double4 vj = D.vm[j];
@@ -891,6 +948,10 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
             for (int d = 0; d < DIM; ++d)
                 for (int e = 0; e < DIM; ++e) T[d * DIM + e] += V * dvn[d] * g.gw[e];
         }
+    }
+    if (LPT > 1) {
+        for (int q = 0; q < DIM * DIM; ++q) T[q] = group_sum<LPT>(T[q]);
+        if (sub) return;
     }
     int ai = mt.x < 0 ? -mt.x : mt.x;
     double rho = D.th[s].x;
@@ -999,31 +1060,46 @@ __global__ __launch_bounds__(kBlock) void k_hsml(Params P, Dev D) {
     D.posh[s] = q;
 }
 
+// Target classes of the general force pass: a launch for every target, or one
+// for fluid targets (|type| < 100) and one for solid targets (|type| > 100),
+// each compiled with only its own terms (fewer live registers, more waves in
+// flight). |type| == 100 (a fluid that artificial stress also acts on) keeps
+// the launch for every target.
+constexpr int kTgtAll = 0, kTgtFluid = 1, kTgtSolid = 2;
+
 // The force pass: everything single_step adds into Delta (singleStep.f90:139-144)
-// plus continuity density and the divergences.
-template <int DIM, typename R>
+// plus continuity density and the divergences. TGT: target class; SPH: 2
+// compiles the SPH = 2 form alone, 0 takes P.pa_sph; LPT: lanes per target
+// (the solid class takes the listed solid slots).
+template <int DIM, typename R, int TGT, int SPH, int LPT>
 __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
-    int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    int sub;
+    const int s = target_slot<LPT>(P, TGT == kTgtSolid ? D.sidx : nullptr, P.n_sidx, sub);
     if (s >= P.n) return;
     int4 mt = D.meta[s];
     if (mt.w >= P.n_real) return;
     const int ti = mt.x;
     const int ai = ti < 0 ? -ti : ti;
-    const bool fluid = ai <= 100;
+    if (TGT == kTgtFluid && ai >= 100) return;
+    if (TGT == kTgtSolid && ai <= 100) return;
+    const bool fluid = TGT == kTgtFluid ? true : TGT == kTgtSolid ? false : ai <= 100;
+    const int pa_sph = SPH ? SPH : P.pa_sph;
     double4 qi = D.posh[s];
     double4 vi = D.vm[s];
     double4 thi = D.th[s];
     const R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
-    const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
+    // kernel gradient correction has solid targets only (create refuses it
+    // with fluid real particles)
+    const double* A = (TGT != kTgtFluid && P.kgc) ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
     R Ei[DIM * DIM], Si[DIM * DIM], Ri[4] = {0, 0, 0, 0};
     const bool visc = P.viscosity && fluid;
     for (int q = 0; q < DIM * DIM; ++q) {
         Ei[q] = visc ? (R)D.Ep[(int64_t)s * DIM * DIM + q] : (R)0.0;
         Si[q] = (!fluid) ? (R)D.Sp[(int64_t)s * DIM * DIM + q] : (R)0.0;
     }
-    const bool as_on = DIM == 2 && P.arti_stress && ti >= 100;
+    const bool as_on = TGT != kTgtFluid && DIM == 2 && P.arti_stress && ti >= 100;
     R wdelta = (R)1.0;
-    if (P.arti_stress && DIM == 2) {
+    if (as_on) {
         for (int q = 0; q < 4; ++q) Ri[q] = (ti >= 100) ? (R)D.asR[(int64_t)s * 4 + q] : (R)0.0;
         R G;
         kernel_wg<DIM, R>(P.skf, (R)P.as_delta, (R)qi.w, f, &wdelta, &G);
@@ -1031,7 +1107,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     R acc[3] = {0.0, 0.0, 0.0};
     R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0, drho_r = 0.0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
-    for (int k = 0; k < n; ++k) {
+    for (int k = sub; k < n; k += LPT) {
         int j = D.nbr[(int64_t)k * P.ldn + s];
         double4 qj = D.posh[j];
         double4 vj = D.vm[j];
@@ -1047,21 +1123,21 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         sdiv_x += (R)thj.w * dxg;
         // internal force, SPH = 1 (internalForce.f90:197-225) and SPH = 3,
         // PVRS Riemann (:277-325): fluids only, solids take none
-        if (P.pa_sph != 2) {
+        if (pa_sph != 2) {
             if (fluid) {
                 const R rhoij = (R)1.0 / (R)(thi.x * thj.x);
                 const R pi = (R)D.p[s], pj = (R)D.p[j];
                 R a13, w[3];
                 R ex[3] = {0, 0, 0}, v_l = 0, v_r = 0;
                 const R vix[3] = {(R)vi.x, (R)vi.y, (R)vi.z}, vjx[3] = {(R)vj.x, (R)vj.y, (R)vj.z};
-                if (P.pa_sph == 3) {
+                if (pa_sph == 3) {
                     for (int d = 0; d < DIM; ++d) {
                         ex[d] = -g.dx[d] / g.r;   // (x_j - x_i) / |x_j - x_i|
                         v_l += vix[d] * ex[d];
                         v_r += vjx[d] * ex[d];
                     }
                 }
-                if (P.pa_sph == 1) {
+                if (pa_sph == 1) {
                     a13 = (pi + pj) * rhoij;
                     for (int d = 0; d < DIM; ++d) w[d] = dv[d];
                 } else {
@@ -1088,7 +1164,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
                 }
                 de_in += mj * a13 * dot;
             }
-            if (P.pa_sph == 3) {
+            if (pa_sph == 3) {
                 // density.f90:109-145 with the pressures of the last EOS and
                 // the neighbour's sound speed in both impedances
                 const R cj = (R)thj.z;
@@ -1190,12 +1266,21 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
             }
         }
     }
+    if (LPT > 1) {
+        for (int d = 0; d < 3; ++d) acc[d] = group_sum<LPT>(acc[d]);
+        de_in = group_sum<LPT>(de_in);
+        de_av = group_sum<LPT>(de_av);
+        sdiv_v = group_sum<LPT>(sdiv_v);
+        sdiv_x = group_sum<LPT>(sdiv_x);
+        drho_r = group_sum<LPT>(drho_r);
+        if (sub) return;
+    }
     double ac[3] = {(double)acc[0], (double)acc[1], (double)acc[2]};
     if (P.gravity && !(P.gravity_mode == SPHGPU_GRAVITY_SKIP_SOLID && ti > 100))
         ac[DIM - 1] += P.g;
     if (mt.z == 1) ac[0] = ac[1] = ac[2] = 0.0;   // fixedBoundary, boundaryCondition.f90:32-45
     double drho = P.summation_density ? 0.0 : thi.x * (double)sdiv_v;
-    if (P.pa_sph == 3) drho = (double)drho_r;
+    if (pa_sph == 3) drho = (double)drho_r;
     D.dv[s] = make_double4(ac[0], ac[1], ac[2], drho);
     double tds = (P.viscosity || !fluid) ? D.tds[s] : 0.0;
     D.de[s] = tds + 0.5 * (double)de_in + (double)de_av;
@@ -1363,6 +1448,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
 }
 
 #include "fluid_fast.hpp"
+#include "general_vlist.hpp"
 
 // Leap-frog correct (timeIntegration.f90:178-252) and bounding-box partials
 // of the new positions for the next step's grid.
@@ -1438,16 +1524,23 @@ __global__ __launch_bounds__(kBlock) void k_correct(Params P0, Dev D, int first_
 }
 
 // Neighbour statistics of the last search (print_statistics, nnps.f90:377-416):
-// out[0] total pairs, out[1] max, out[2] min, out[3] targets with none.
+// out[0] total pairs, out[1] max, out[2] min, out[3] targets with none. A
+// fixed grid of kStatsBlocks blocks strides over the particles and each block
+// adds its results with one atomic per word: same-address atomics serialise
+// in L2, so there are few of them.
+constexpr int kStatsBlocks = 256;
+
 __global__ __launch_bounds__(kBlock) void k_stats(Params P, Dev D, const int* counts,
                                                   unsigned long long* out) {
-    int s = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long tot = 0, mx = 0, mn = ~0ull, zero = 0;
-    if (s < P.n) {
-        int4 mt = D.meta[s];
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+        const int4 mt = D.meta[s];
         if (mt.w < P.n_real) {
-            unsigned long long c = (mt.y == -1) ? 0 : (unsigned long long)counts[s];
-            tot = c; mx = c; mn = c; zero = c == 0;
+            const unsigned long long c = (mt.y == -1) ? 0 : (unsigned long long)counts[s];
+            tot += c;
+            mx = c > mx ? c : mx;
+            mn = c < mn ? c : mn;
+            zero += c == 0;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -1457,7 +1550,23 @@ __global__ __launch_bounds__(kBlock) void k_stats(Params P, Dev D, const int* co
         mx = a > mx ? a : mx;
         mn = b < mn ? b : mn;
     }
+    constexpr int kWaves = kBlock / 64;
+    __shared__ unsigned long long red[4][kWaves];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        red[0][w] = tot;
+        red[1][w] = mx;
+        red[2][w] = mn;
+        red[3][w] = zero;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kWaves; ++k) {
+            tot += red[0][k];
+            mx = red[1][k] > mx ? red[1][k] : mx;
+            mn = red[2][k] < mn ? red[2][k] : mn;
+            zero += red[3][k];
+        }
         atomicAdd(&out[0], tot);
         atomicMax(&out[1], mx);
         atomicMin(&out[2], mn);
@@ -1497,6 +1606,7 @@ struct HostRun {
     std::vector<uint32_t> sort_key, sort_start;   // counting-sort work space
     std::vector<int64_t> order;                   // slot -> particle index
     std::vector<int32_t> type_idx;                // type by particle index
+    std::vector<int> sidx;                        // slots of the real solid targets
 };
 
 // Host staging for downloads: pinned (hipHostMalloc, grown on demand, kept
@@ -1539,6 +1649,10 @@ struct sphgpu_ctx {
     int64_t n = 0, n_real = 0, n_dummy = 0;
     int dim = 2;
     hipStream_t stream = nullptr;
+    // the general force pass of solid targets runs beside that of fluid
+    // targets: a second stream, forked from and joined to `stream` by events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     Dev D{};
     std::vector<void*> allocs;
     std::vector<size_t> alloc_bytes;   // size of each allocation (checkpoints copy them whole)
@@ -1553,8 +1667,8 @@ struct sphgpu_ctx {
     int64_t last_force_launches = 0;
     bool time_force = false;
     bool force_general = false;   // testing switch: use the general force kernel
-    bool use_verlet = true;       // fluid fast path: candidate list + fused pass
-    bool fused_done = false;      // a fused pass has run since the upload: candidate list valid
+    bool use_verlet = true;       // candidate lists (SPHGPU_NO_VERLET=1: exact lists by a cell sweep every step)
+    bool fused_done = false;      // a step has run since the upload: the candidate list is valid
     bool use_graph = true;        // replay captured step sequences (SPHGPU_NO_GRAPH=1: eager)
     hipGraphExec_t graph = nullptr;   // kGraphSteps fast-path steps, valid until the next upload
     std::string err;
@@ -1707,6 +1821,18 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
             return create_fail(c, SPHGPU_E_HIP, hip_why("hipEventCreate", e));
         }
     }
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        c->side = nullptr;
+        return create_fail(c, SPHGPU_E_HIP, hip_why("hipStreamCreateWithFlags", e));
+    }
+    for (hipEvent_t* ev : {&c->fork, &c->join}) {
+        e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            *ev = nullptr;
+            return create_fail(c, SPHGPU_E_HIP, hip_why("hipEventCreateWithFlags", e));
+        }
+    }
     e = hipHostMalloc((void**)&c->h_err, sizeof(int) * 4, hipHostMallocDefault);
     if (e != hipSuccess) {
         c->h_err = nullptr;
@@ -1840,7 +1966,7 @@ static void plan_grid_params(Params& P, int dim, bool verlet, int64_t n, const d
 }
 
 static void plan_grid(sphgpu_ctx* c, const double* x, const double* h) {
-    plan_grid_params(c->P, c->dim, fluid_fast_path(c) && c->use_verlet, c->n, x, h, env_tuning());
+    plan_grid_params(c->P, c->dim, c->use_verlet, c->n, x, h, env_tuning());
 }
 
 template <int DIM>
@@ -1943,20 +2069,21 @@ static void drop_graph(sphgpu_ctx* c) {
     c->graph = nullptr;
 }
 
-// Capture kGraphSteps body steps (kernel nodes only: no memsets, no events).
-// The kernels take the uploaded Params and device arrays by value, and a body
-// step after the first has the same arguments every time, so the graph holds
-// until the next upload. Capturing launches nothing, so it is done on the
-// first advance after an upload whatever its length (the steps are captured
-// as steps after the first), and no later call pays for it.
-template <int DIM>
-static int capture_body(sphgpu_ctx* c) {
+// Capture kGraphSteps steps of `body` (kernel nodes only: no memsets, no
+// events). The kernels take the uploaded Params and device arrays by value,
+// and a step after the first has the same arguments every time, so the graph
+// holds until the next upload. Capturing launches nothing, so it is done on
+// the first advance after an upload whatever its length (the steps are
+// captured as steps after the first), and no later call pays for it.
+template <typename F>
+static int capture_steps(sphgpu_ctx* c, F&& body) {
     const int64_t step = c->step;
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->step = std::max<int64_t>(step, 2);
+    c->time_force = false;
     for (int k = 0; k < kGraphSteps; ++k) {
-        fast_body_step<DIM>(c, false);
+        body();
         c->step++;
     }
     hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -1978,7 +2105,7 @@ static int capture_body(sphgpu_ctx* c) {
 template <int DIM>
 static int advance_fast(sphgpu_ctx* c, int nsteps) {
     if (c->use_graph && !c->graph) {
-        int rc = capture_body<DIM>(c);
+        int rc = capture_steps(c, [c] { fast_body_step<DIM>(c, false); });
         if (rc) return rc;
     }
     launch_fast_head<DIM>(c, c->step == 1 ? 1 : 0);
@@ -2015,6 +2142,76 @@ static void launch_point_build(sphgpu_ctx* c, int mode) {
         hipLaunchKernelGGL((k_point_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, mode);
 }
 
+// The general path's pair passes before the force pass: kernel gradient
+// correction, summation density, strain rates, with LPT lanes per target.
+template <int DIM, typename R, int LPT>
+static void launch_pair_passes(sphgpu_ctx* c) {
+    const Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const int nbl = (int)(((int64_t)P.n * LPT + kBlock - 1) / kBlock);
+    if (P.kgc) hipLaunchKernelGGL((k_kgc<DIM, R, LPT>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+    if (P.summation_density) {
+        hipLaunchKernelGGL((k_sumdens<DIM, R, LPT>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL(k_apply_sumdens, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    }
+    if (P.viscosity || P.has_solid)
+        hipLaunchKernelGGL((k_strain<DIM, R, LPT>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+}
+
+// The general force pass: one launch per target class present (fluid, solid),
+// each with the SPH = 2 form compiled alone when that is the run's form.
+template <int DIM, typename R, int SPH>
+static void launch_force_classes(sphgpu_ctx* c) {
+    const Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    if (P.n_real <= kSmallRun) {
+        const int nbl = (int)(((int64_t)P.n * kLanesPerTarget + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtAll, SPH, kLanesPerTarget>), dim3(nbl), dim3(kBlock), 0, c->stream,
+                           P, c->D);
+        return;
+    }
+    if (P.has_t100) {
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtAll, SPH, 1>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        return;
+    }
+    // the solid targets are few: they take several lanes each, on the side
+    // stream while the fluid pass runs
+    const bool solids = P.n_sidx > 0;
+    if (solids) {
+        const int nbs = (int)(((int64_t)P.n_sidx * kLanesPerTarget + kBlock - 1) / kBlock);
+        hipEventRecord(c->fork, c->stream);
+        hipStreamWaitEvent(c->side, c->fork, 0);
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtSolid, SPH, kLanesPerTarget>), dim3(nbs), dim3(kBlock), 0, c->side,
+                           P, c->D);
+        hipEventRecord(c->join, c->side);
+    }
+    hipLaunchKernelGGL((k_force<DIM, R, kTgtFluid, SPH, 1>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (solids) hipStreamWaitEvent(c->stream, c->join, 0);
+}
+
+template <int DIM, typename R>
+static void launch_force_general(sphgpu_ctx* c) {
+    if (c->P.pa_sph == 2) launch_force_classes<DIM, R, 2>(c);
+    else launch_force_classes<DIM, R, 0>(c);
+}
+
+// The general path's search with a candidate list: reference grid, predict +
+// EOS + reference cells + displacement vote, the rebuild (cell sort and
+// candidate list) when the gate is open, and this step's exact list from the
+// candidates. count_only: size the candidate list (no exact list).
+template <int DIM>
+static void launch_gen_search(sphgpu_ctx* c, int mode, int count_only) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const Gate G = fast_gate(c);
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+    hipLaunchKernelGGL(k_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode);
+    launch_sort<DIM>(c, G, 0);
+    hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only,
+                       RunMap{});
+    if (!count_only) hipLaunchKernelGGL(k_filter<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+}
+
 template <int DIM>
 static void launch_derivs(sphgpu_ctx* c, int mode) {
     Params& P = c->P;
@@ -2024,23 +2221,21 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
         launch_fast<DIM>(c, mode);
         return;
     }
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
-    launch_sort<DIM>(c);
-    launch_point_build<DIM>(c, mode);
+    if (c->use_verlet) {
+        launch_gen_search<DIM>(c, mode, 0);
+    } else {
+        hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+        launch_sort<DIM>(c);
+        launch_point_build<DIM>(c, mode);
+    }
     if (!fast) {
         // pair32: the pair sums of the general path in FP32 (mixed precision)
-        if (P.kgc) {
-            if (P.pair32) hipLaunchKernelGGL((k_kgc<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-            else hipLaunchKernelGGL((k_kgc<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-        }
-        if (P.summation_density) {
-            if (P.pair32) hipLaunchKernelGGL((k_sumdens<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-            else hipLaunchKernelGGL((k_sumdens<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-            hipLaunchKernelGGL(k_apply_sumdens, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-        }
-        if (P.viscosity || P.has_solid) {
-            if (P.pair32) hipLaunchKernelGGL((k_strain<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-            else hipLaunchKernelGGL((k_strain<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        if (P.n_real <= kSmallRun) {
+            if (P.pair32) launch_pair_passes<DIM, float, kLanesPerTarget>(c);
+            else launch_pair_passes<DIM, double, kLanesPerTarget>(c);
+        } else {
+            if (P.pair32) launch_pair_passes<DIM, float, 1>(c);
+            else launch_pair_passes<DIM, double, 1>(c);
         }
         if (P.has_solid)
             hipLaunchKernelGGL(k_solid_point<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
@@ -2055,9 +2250,9 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
     else if (fast)
         hipLaunchKernelGGL((k_force_fluid<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     else if (P.pair32)
-        hipLaunchKernelGGL((k_force<DIM, float>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        launch_force_general<DIM, float>(c);
     else
-        hipLaunchKernelGGL((k_force<DIM, double>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        launch_force_general<DIM, double>(c);
     if (c->time_force) hipEventRecord(c->ev[3], c->stream);
     if (P.sle) hipLaunchKernelGGL(k_hsml<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
 }
@@ -2071,6 +2266,31 @@ static void launch_step(sphgpu_ctx* c) {
         hipLaunchKernelGGL((k_correct<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first, RunMap{}, 0);
     else
         hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first, RunMap{}, 0);
+}
+
+// nsteps steps of the general path: the first step of a run eager, then
+// whole groups of kGraphSteps replaying a captured graph while more than one
+// step is left, and the rest eager, the last one with its force pass timed.
+template <int DIM>
+static int advance_general(sphgpu_ctx* c, int nsteps) {
+    if (c->use_graph && !c->graph) {
+        int rc = capture_steps(c, [c] { launch_step<DIM>(c); });
+        if (rc) return rc;
+    }
+    for (int k = 0; k < nsteps;) {
+        if (c->use_graph && c->step > 1 && nsteps - k > kGraphSteps) {
+            HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+            c->step += kGraphSteps;
+            k += kGraphSteps;
+            continue;
+        }
+        c->time_force = (k == nsteps - 1);
+        launch_step<DIM>(c);
+        c->step++;
+        ++k;
+    }
+    c->time_force = false;
+    return 0;
 }
 
 static int check_err(sphgpu_ctx* c) {
@@ -2115,6 +2335,9 @@ static int alloc_state(sphgpu_ctx* c) {
     rc |= dalloc(c, &D.perm, N); rc |= dalloc(c, &D.ncnt, N);
     rc |= dalloc(c, &D.cpos, N);
     rc |= dalloc(c, &D.vcnt, N); rc |= dalloc(c, &D.ecnt, N);
+    // FP32 position records: the mixed fast path's pair inputs, and the
+    // pre-test of every candidate path
+    if (c->P.mixed || c->use_verlet) rc |= dalloc(c, &D.posf, N);
     rc |= dalloc(c, &D.vflag, 2);
     bool need_tensor = c->cfg.viscosity || c->P.has_solid;
     if (need_tensor) rc |= dalloc(c, &D.Ep, T);
@@ -2125,11 +2348,11 @@ static int alloc_state(sphgpu_ctx* c) {
         rc |= dalloc(c, &D.Ed, T); rc |= dalloc(c, &D.rot, T);
         rc |= dalloc(c, &D.shear_prev, T); rc |= dalloc(c, &D.Sp, T);
         rc |= dalloc(c, &D.asR, N * 4);
+        rc |= dalloc(c, &D.sidx, N);
     }
     if (c->cfg.kernel_correction) rc |= dalloc(c, &D.kgcA, T);
     if (c->P.mixed) {
         rc |= dalloc(c, &D.prf, N); rc |= dalloc(c, &D.drho, N);
-        rc |= dalloc(c, &D.posf, N);
         rc |= dalloc(c, &D.drho_prev, N); rc |= dalloc(c, &D.dvf, N); rc |= dalloc(c, &D.def, N);
     }
     if (c->cfg.summation_density) rc |= dalloc(c, &D.rho_new, N);
@@ -2183,6 +2406,13 @@ static int alloc_nbr(sphgpu_ctx* c, int cap) {
     return 0;
 }
 
+// Testing switch SPHGPU_VERLET_CAP=k: at most k candidate rows per particle,
+// so that particles past it take the overflow walk of their cell window.
+static int test_verlet_cap(int cap) {
+    const char* v = getenv("SPHGPU_VERLET_CAP");
+    return v ? std::min(cap, std::max(1, atoi(v))) : cap;
+}
+
 template <int DIM>
 static int size_neighbour_list(sphgpu_ctx* c) {
     // count-only passes on the uploaded state, then capacities with headroom
@@ -2215,7 +2445,26 @@ static int size_neighbour_list(sphgpu_ctx* c) {
         e = check_err(c);
         if (e) return e;
         int vx = c->h_err[1];
-        int vcap = vx + std::max(8, vx / 2);
+        int vcap = test_verlet_cap(vx + std::max(8, vx / 2));
+        c->P.verlet_cap = vcap;
+        const int64_t need = vlist_entries(c->capacity, vcap);
+        if (need > c->vnbr_alloc) {
+            if (dalloc(c, &c->D.vnbr, (size_t)need)) return SPHGPU_E_HIP;
+            c->vnbr_alloc = need;
+        }
+    } else if (c->use_verlet) {
+        HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
+        force_rebuild(c);
+        launch_gen_search<DIM>(c, 2 | 4, 1);
+        HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        e = check_err(c);
+        if (e) return e;
+        int vx = c->h_err[1];
+        int vcap = test_verlet_cap(vx + std::max(8, vx / 2));
+        // an evolving h: the list is built for the planned largest h, whose
+        // candidate count the sizing pass (at the uploaded h) has taken
+        // already; the headroom covers the particles crowding in
         c->P.verlet_cap = vcap;
         const int64_t need = vlist_entries(c->capacity, vcap);
         if (need > c->vnbr_alloc) {
@@ -2271,8 +2520,11 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     Params& P = hr.P;
     fill_params_cfg(P, cfg, n, n_real);
     P.has_solid = 0;
-    for (int64_t i = 0; i < n; ++i)
+    P.has_t100 = 0;
+    for (int64_t i = 0; i < n; ++i) {
         if (type[i] > 100 || type[i] < -100) P.has_solid = 1;
+        if (type[i] == 100 || type[i] == -100) P.has_t100 = 1;
+    }
     {
         bool multi = false;
         int first_pos = 0;
@@ -2303,7 +2555,7 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     if (P.sle == 1)   // hsml.f90:22-25, with the uploaded density
         for (int64_t i = 0; i < n_real; ++i)
             if (rho[i] > 0) h_next_max = std::max(h_next_max, 2.0 * std::pow(mass[i] / rho[i], 1.0 / dim));
-    plan_grid_params(P, dim, hr.fast && use_verlet, n, x, h, tune, h_next_max);
+    plan_grid_params(P, dim, use_verlet, n, x, h, tune, h_next_max);
     // slots by cell key, ascending particle index within a cell: a stable
     // counting sort over the P.ncell (< 2^31) cells
     std::vector<uint32_t>& key = hr.sort_key;
@@ -2359,6 +2611,15 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
                 for (int q = 0; q < dim * dim; ++q) hr.shear[s * dim * dim + q] = shear[i * dim * dim + q];
         }
     });
+    // slots of the real solid targets (the solid launch of the general force
+    // pass covers these alone; slots do not move after upload)
+    hr.sidx.clear();
+    if (P.has_solid)
+        for (int64_t s = 0; s < n; ++s) {
+            const int4 mt = hr.meta[s];
+            if (mt.w < n_real && (mt.x > 100 || mt.x < -100)) hr.sidx.push_back((int)s);
+        }
+    P.n_sidx = (int)hr.sidx.size();
     if (P.mixed) {
         hr.drho.resize(n);
         for (int64_t s = 0; s < n; ++s) {
@@ -2423,6 +2684,9 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     if (!vshear.empty())
         HIPCHK(c, hipMemcpyAsync(D.shear, vshear.data(), vshear.size() * sizeof(double), hipMemcpyHostToDevice,
                                  c->stream));
+    if (!hr.sidx.empty())
+        HIPCHK(c, hipMemcpyAsync(D.sidx, hr.sidx.data(), hr.sidx.size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c->stream));
     if (c->P.mixed) {
         HIPCHK(c, hipMemcpyAsync(D.drho, hr.drho.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2455,7 +2719,7 @@ extern "C" int sphgpu_derivatives(sphgpu_ctx* c) {
     if (!c->fused_done) force_rebuild(c);
     if (c->dim == 2) launch_derivs<2>(c, 2);
     else launch_derivs<3>(c, 2);
-    c->fused_done = fluid_fast_path(c) && c->use_verlet;
+    c->fused_done = c->use_verlet;
     HIPCHK(c, hipGetLastError());
     return check_err(c);
 }
@@ -2467,7 +2731,7 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
     // refresh the grid partials; build the candidate list if none is valid
     hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     if (!c->fused_done) force_rebuild(c);
-    if (nsteps > 0) c->fused_done = fluid_fast_path(c) && c->use_verlet;
+    if (nsteps > 0) c->fused_done = c->use_verlet;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     double force_ms = 0;
     int64_t force_n = 0;
@@ -2475,13 +2739,8 @@ extern "C" int sphgpu_advance(sphgpu_ctx* c, int32_t nsteps) {
         int rc = c->dim == 2 ? advance_fast<2>(c, nsteps) : advance_fast<3>(c, nsteps);
         if (rc) return rc;
     } else {
-        for (int k = 0; k < nsteps; ++k) {
-            c->time_force = (k == nsteps - 1);
-            if (c->dim == 2) launch_step<2>(c);
-            else launch_step<3>(c);
-            c->step++;
-        }
-        c->time_force = false;
+        int rc = c->dim == 2 ? advance_general<2>(c, nsteps) : advance_general<3>(c, nsteps);
+        if (rc) return rc;
     }
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, hipGetLastError());
@@ -2821,7 +3080,7 @@ extern "C" int sphgpu_neighbour_stats(sphgpu_ctx* c, int64_t* out4) {
     unsigned long long init[4] = {0, 0, ~0ull, 0};
     HIPCHK(c, hipMalloc(&d, sizeof(init)));
     HIPCHK(c, hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
-    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    const int nb = std::min((int)((c->P.n + kBlock - 1) / kBlock), kStatsBlocks);
     const int* counts = (fluid_fast_path(c) && c->use_verlet && c->fused_done) ? c->D.ecnt : c->D.ncnt;
     hipLaunchKernelGGL(k_stats, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, counts, d);
     unsigned long long h[4];
@@ -2872,7 +3131,13 @@ extern "C" void sphgpu_destroy(sphgpu_ctx* c) {
     if (c->dorder) hipFree(c->dorder);
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
+    if (c->fork) hipEventDestroy(c->fork);
+    if (c->join) hipEventDestroy(c->join);
     if (c->h_err) hipHostFree(c->h_err);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -14,10 +14,13 @@ changes sign, so the run amplifies last-bit changes: over 20,000 steps a
 stress field by 34 % and its tip deflection by 1.6e-4. The fixture records
 that response, and a FP64 result is held to max(1e-9, 10 x response).
 
-Mixed precision rounds every pair term to FP32 (~6e-8). Bounds: every field
-within 1e-5 of FP64 after 1,000 steps (before the amplification sets in);
-after 20,000 steps the tip deflection within 1e-3, positions within 1e-5 and
-density within 1e-3, with the pointwise and root bending stress reported.
+Mixed precision rounds every pair term to FP32 (~6e-8). Bounds: after 1,000
+steps every field within max(1e-5, 10 x the FP64 run's own response to a
+1e-15 perturbation of the initial velocities) of FP64 (the amplification
+has started by then: a change of summation order alone moves the FP64
+pressure by ~2.5e-4); after 20,000 steps the tip deflection within 1e-3,
+positions within 1e-5 and density within 1e-3, with the pointwise and root
+bending stress reported.
 """
 from __future__ import annotations
 
@@ -113,10 +116,15 @@ def test_elastic_beam_mixed_vs_fp64():
     nt = case["n_real"]
     s64, o64 = _run(case, C.FP64, 20000, 500, snap=(1000,))
     s32, o32 = _run(case, C.MIXED, 20000, 500, snap=(1000,))
+    pert = dict(case)
+    pert["v"] = case["v"] * (1 + 1e-15 * np.random.default_rng(7).standard_normal(case["v"].shape))
+    _, p64 = _run(pert, C.FP64, 1000, 500)
+    sens = {f: rel_err(p64[1000][f][:nt], o64[1000][f][:nt]) for f in FIELDS}
     early = {f: rel_err(o32[1000][f][:nt], o64[1000][f][:nt]) for f in FIELDS}
+    print("FP64 response to a 1e-15 perturbation after 1000 steps:", {k: "%.2e" % v for k, v in sens.items()})
     print("mixed vs FP64 after 1000 steps:", {k: "%.2e" % v for k, v in early.items()})
     for f in FIELDS:
-        assert early[f] <= 1e-5, (f, early[f])
+        assert early[f] <= max(1e-5, 10 * sens[f]), (f, early[f], sens[f])
     a, b = o32[20000], o64[20000]
     late = {f: rel_err(a[f][:nt], b[f][:nt]) for f in FIELDS}
     late["tip"] = rel_err(s32, s64)
@@ -132,20 +140,26 @@ def test_elastic_beam_mixed_vs_fp64():
 @pytest.mark.parametrize("which,asw", [("2d", 0), ("2d", 1), ("3d", 0)])
 def test_reference_beams_mixed_vs_fp64(which, asw):
     """The reference's own beams (kernel-gradient correction on) in mixed
-    precision against FP64 after 20 steps."""
+    precision against FP64 after 20 steps: within max(1e-5, 10 x the FP64
+    run's own response to a 1e-15 perturbation of the initial velocities)
+    (the artificial-stress rotation amplifies last bits, as above)."""
     from tests.golden.make_golden import beam_velocity
     case = cases.reference_can_beam() if which == "2d" else cases.reference_can_beam_3d()
     case["v"] = beam_velocity(case["x"])
     nt = case["n_real"]
+    pert = dict(case)
+    pert["v"] = case["v"] * (1 + 1e-15 * np.random.default_rng(1).standard_normal(case["v"].shape))
     out = {}
-    for prec in (C.FP64, C.MIXED):
+    for key, c, prec in (("fp64", case, C.FP64), ("mixed", case, C.MIXED), ("pert", pert, C.FP64)):
         cfg = case["config"].copy()
         cfg.artificial_stress = asw
         cfg.precision = prec
-        with Simulation(case, cfg) as sim:
+        with Simulation(c, cfg) as sim:
             sim.advance(20)
-            out[prec] = {f: sim.get(f)[:nt] for f in FIELDS}
-    err = {f: rel_err(out[C.MIXED][f], out[C.FP64][f]) for f in FIELDS}
-    print(which, asw, {k: "%.2e" % v for k, v in err.items()})
+            out[key] = {f: sim.get(f)[:nt] for f in FIELDS}
+    err = {f: rel_err(out["mixed"][f], out["fp64"][f]) for f in FIELDS}
+    sens = {f: rel_err(out["pert"][f], out["fp64"][f]) for f in FIELDS}
+    print(which, asw, "mixed vs FP64", {k: "%.2e" % v for k, v in err.items()},
+          "FP64 response", {k: "%.2e" % v for k, v in sens.items()})
     for f in FIELDS:
-        assert err[f] <= 1e-5, (f, err[f])
+        assert err[f] <= max(1e-5, 10 * sens[f]), (f, err[f], sens[f])
