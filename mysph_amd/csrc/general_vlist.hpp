@@ -14,10 +14,10 @@
 // smoothing-length update enforces (k_hsml).
 
 // Predict + EOS (as k_point_build), the FP32 position record of the filter's
-// pre-test, the reference cells of this step when the reference window can
-// exclude a pair, and the displacement vote for a rebuild. mode bit 0: first
-// step (no predict); bit 1: no predict at all; bit 2: sizing pass (the state
-// is left as uploaded).
+// pre-test (and in mixed precision the FP32 pair record), the reference
+// cells of this step when the reference window can exclude a pair, and the
+// displacement vote for a rebuild. mode bit 0: first step (no predict); bit
+// 1: no predict at all; bit 2: sizing pass (the state is left as uploaded).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_point_gen(Params P, Dev D, int mode) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
@@ -32,6 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_point_gen(Params P, Dev D, int mode)
         }
         const double4 q = D.posh[s];
         D.posf[s] = cell_offset_record(P, q);
+        if (P.pair32) write_rec32(D, s);
         if (mt.y != -1 && P.grid->need_win) {
             const RefGrid g = *P.grid;
             bool bad = false;

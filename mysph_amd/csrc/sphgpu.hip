@@ -114,6 +114,8 @@ struct Dev {
     int* ecnt;        // accepted neighbours per particle in the last force pass
     unsigned long long* vflag;   // rebuild gate (see Gate)
     int* sidx;        // general path: the slots of the real solid targets, in slot order
+    float* Epf;       // mixed general path (pair32): FP32 copies of Ep and Sp for the pair gathers
+    float* Spf;
 };
 
 // Block min/max of positions -> partial[blockIdx*6 + (0..2 min, 3..5 max)].
@@ -642,6 +644,35 @@ __device__ __forceinline__ float4 cell_offset_record(const Params& P, const doub
     return make_float4(off[0], off[1], off[2], __uint_as_float(pk));
 }
 
+template <int DIM>
+__device__ __forceinline__ void unpack_cell(uint32_t pk, int& x, int& y, int& z) {
+    if (DIM == 3) { x = (int)(pk & 2047u); y = (int)((pk >> 11) & 2047u); z = (int)((pk >> 22) & 511u); }
+    else { x = (int)(pk & 65535u); y = (int)((pk >> 16) & 32767u); z = 0; }
+}
+
+// x_i - x_j in FP32 from two cell-offset records: (off_i - off_j) + (c_i -
+// c_j) cs, the cell part exact.
+template <int DIM>
+__device__ __forceinline__ void rec_dx(const float4& pi, const float4& pj, float cs, float dx[3]) {
+    int cxi, cyi, czi, cxj, cyj, czj;
+    unpack_cell<DIM>(__float_as_uint(pi.w), cxi, cyi, czi);
+    unpack_cell<DIM>(__float_as_uint(pj.w), cxj, cyj, czj);
+    dx[0] = (pi.x - pj.x) + (float)(cxi - cxj) * cs;
+    dx[1] = (pi.y - pj.y) + (float)(cyi - cyj) * cs;
+    dx[2] = DIM == 3 ? (pi.z - pj.z) + (float)(czi - czj) * cs : 0.0f;
+}
+
+// FP32 pair record of the general path in mixed precision (pair32): velocity
+// and volume m / rho, then rho, p / rho^2, c and h, from the FP64 state.
+__device__ __forceinline__ void write_rec32(const Dev& D, int s) {
+    const double4 v = D.vm[s];
+    const double4 t = D.th[s];
+    PairRecF rec;
+    rec.vm = make_float4((float)v.x, (float)v.y, (float)v.z, (float)t.w);
+    rec.th = make_float4((float)t.x, (float)t.y, (float)t.z, (float)D.posh[s].w);
+    D.prf[s] = rec;
+}
+
 __device__ __forceinline__ void eos_mixed(const Params& P, const Dev& D, int s, int type) {
     eos_mixed_vals(P, D, s, type, D.drho[s], D.vm[s], D.posh[s]);
 }
@@ -671,6 +702,10 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
         if (!(mode & 4)) {
             if (P.pa_sph == 3) D.pold[s] = D.p[s];
             if (!P.summation_density) eos_th(P, D, s, mt);
+        }
+        if (P.pair32) {
+            D.posf[s] = cell_offset_record(P, D.posh[s]);
+            write_rec32(D, s);
         }
     }
     if (!real || mt.y == -1) {
@@ -818,24 +853,31 @@ struct PairGeo {
     R r2, r, w, gw[3];
 };
 
+// r, W and grad W of a pair whose g.dx is set (hi: the target's h).
 template <int DIM, typename R>
-__device__ __forceinline__ void pair_geo(const Params& P, const double4& qi, const double4& qj,
-                                         R factor, const double* A, PairGeo<DIM, R>& g) {
-    g.dx[0] = (R)(qi.x - qj.x);
-    g.dx[1] = (R)(qi.y - qj.y);
-    g.dx[2] = DIM == 3 ? (R)(qi.z - qj.z) : (R)0.0;
+__device__ __forceinline__ void pair_geo_dx(const Params& P, R hi, R factor, const double* A,
+                                            PairGeo<DIM, R>& g) {
     R dr = g.dx[0] * g.dx[0] + g.dx[1] * g.dx[1];
     if (DIM == 3) dr += g.dx[2] * g.dx[2];
     g.r2 = dr;
     g.r = sqrt(dr);
     R G;
-    kernel_wg<DIM, R>(P.skf, g.r, (R)qi.w, factor, &g.w, &G);
+    kernel_wg<DIM, R>(P.skf, g.r, hi, factor, &g.w, &G);
     for (int d = 0; d < 3; ++d) g.gw[d] = (d < DIM) ? G * g.dx[d] : (R)0.0;
     if (A) {
         double b[3] = {(double)g.gw[0], (double)g.gw[1], (double)g.gw[2]};
         mcpe_solve<DIM>(A, b);
         for (int d = 0; d < 3; ++d) g.gw[d] = (R)b[d];
     }
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void pair_geo(const Params& P, const double4& qi, const double4& qj,
+                                         R factor, const double* A, PairGeo<DIM, R>& g) {
+    g.dx[0] = (R)(qi.x - qj.x);
+    g.dx[1] = (R)(qi.y - qj.y);
+    g.dx[2] = DIM == 3 ? (R)(qi.z - qj.z) : (R)0.0;
+    pair_geo_dx<DIM, R>(P, (R)qi.w, factor, A, g);
 }
 
 // kernelGradientCorrection.f90:20-35: A_i = sum_j V_j (x_j - x_i)(e) dW(d).
@@ -915,6 +957,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
     int4 mt = D.meta[s];
     if (mt.w < P.n_real) D.rho[s] = D.rho_new[s];
     eos_th(P, D, s, mt);
+    if (P.pair32) write_rec32(D, s);
 }
 
 // Strain-rate (and solid rotation-rate) sums, internalForce.f90:48-100, and the
@@ -932,19 +975,38 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
     for (int q = 0; q < DIM * DIM; ++q) T[q] = (R)0.0;
     bool real = mt.w < P.n_real && mt.y != -1;
     if (real) {
+        // mixed precision (R = float) gathers the FP32 pair records
+        constexpr bool REC32 = std::is_same<R, float>::value;
         double4 qi = D.posh[s];
         double4 vi = D.vm[s];
         R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
         const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
+        float4 pfi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (REC32) pfi = D.posf[s];
+        const float csf = (float)P.cs;
         int n = min(D.ncnt[s], P.nbr_cap);
         for (int k = sub; k < n; k += LPT) {
             int j = D.nbr[(int64_t)k * P.ldn + s];
-            double4 qj = D.posh[j];
-            double4 vj = D.vm[j];
             PairGeo<DIM, R> g;
-            pair_geo<DIM, R>(P, qi, qj, f, A, g);
-            R V = (R)(vj.w / D.th[j].x);
-            R dvn[3] = {(R)(vj.x - vi.x), (R)(vj.y - vi.y), (R)(vj.z - vi.z)};
+            R V, dvn[3];
+            if constexpr (REC32) {
+                const float4 pfj = D.posf[j];
+                const float4 vj = D.prf[j].vm;
+                rec_dx<DIM>(pfi, pfj, csf, g.dx);
+                pair_geo_dx<DIM, R>(P, (R)qi.w, f, A, g);
+                V = vj.w;
+                dvn[0] = vj.x - (R)vi.x;
+                dvn[1] = vj.y - (R)vi.y;
+                dvn[2] = vj.z - (R)vi.z;
+            } else {
+                double4 qj = D.posh[j];
+                double4 vj = D.vm[j];
+                pair_geo<DIM, R>(P, qi, qj, f, A, g);
+                V = (R)(vj.w / D.th[j].x);
+                dvn[0] = (R)(vj.x - vi.x);
+                dvn[1] = (R)(vj.y - vi.y);
+                dvn[2] = (R)(vj.z - vi.z);
+            }
             for (int d = 0; d < DIM; ++d)
                 for (int e = 0; e < DIM; ++e) T[d * DIM + e] += V * dvn[d] * g.gw[e];
         }
@@ -965,6 +1027,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
                 double E = 2.0 * (double)T[d * DIM + e] - (d == e ? (2.0 / 3.0) * tr : 0.0);
                 sq += E * E;
                 D.Ep[o + d * DIM + e] = P.pa_sph == 2 ? eta * E / (rho * rho) : eta * E;
+                if (P.pair32) D.Epf[o + d * DIM + e] = (float)D.Ep[o + d * DIM + e];
             }
         D.tds[s] = 0.5 * eta / rho * sq;
     } else {
@@ -974,6 +1037,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
                 D.Ed[o + d * DIM + e] = E;                                                 // edot
                 D.rot[o + d * DIM + e] = 0.5 * ((double)T[d * DIM + e] - (double)T[e * DIM + d]);  // rdot
                 D.Ep[o + d * DIM + e] = P.pa_sph == 2 ? eta * E / (rho * rho) : eta * E;
+                if (P.pair32) D.Epf[o + d * DIM + e] = (float)D.Ep[o + d * DIM + e];
             }
     }
 }
@@ -1019,8 +1083,10 @@ __global__ __launch_bounds__(kBlock) void k_solid_point(Params P, Dev D) {
             sig[d * DIM + e] = S[d * DIM + e] - (d == e ? p : 0.0);
             D.stress[o + d * DIM + e] = sig[d * DIM + e];
             D.Sp[o + d * DIM + e] = sig[d * DIM + e] / (rho * rho);
+            if (P.pair32) D.Spf[o + d * DIM + e] = (float)(sig[d * DIM + e] / (rho * rho));
         }
     D.th[s] = make_double4(rho, p / (rho * rho), t.z, t.w);
+    if (P.pair32) write_rec32(D, s);
     if (DIM == 2 && P.arti_stress && mt.x >= 100) {
         double a = sig[0] - sig[3];
         double theta = (a == 0.0) ? 0.5 * kPi : 0.5 * atan((sig[1] + sig[2]) / a);
@@ -1073,6 +1139,10 @@ constexpr int kTgtAll = 0, kTgtFluid = 1, kTgtSolid = 2;
 // (the solid class takes the listed solid slots).
 template <int DIM, typename R, int TGT, int SPH, int LPT>
 __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
+    // mixed precision (R = float) gathers the FP32 pair records; FP64 the
+    // FP64 state. Every per-pair value is an R, so the FP64 arithmetic is
+    // exactly that of the FP64 records.
+    constexpr bool REC32 = std::is_same<R, float>::value;
     int sub;
     const int s = target_slot<LPT>(P, TGT == kTgtSolid ? D.sidx : nullptr, P.n_sidx, sub);
     if (s >= P.n) return;
@@ -1084,10 +1154,16 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     if (TGT == kTgtSolid && ai <= 100) return;
     const bool fluid = TGT == kTgtFluid ? true : TGT == kTgtSolid ? false : ai <= 100;
     const int pa_sph = SPH ? SPH : P.pa_sph;
-    double4 qi = D.posh[s];
-    double4 vi = D.vm[s];
-    double4 thi = D.th[s];
-    const R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
+    const double4 qi = D.posh[s];
+    const double4 vi = D.vm[s];
+    const double4 thi = D.th[s];
+    const R hi = (R)qi.w;
+    const R vi_r[3] = {(R)vi.x, (R)vi.y, (R)vi.z};
+    const R rhoi = (R)thi.x, pri = (R)thi.y, ci = (R)thi.z;
+    float4 pfi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (REC32) pfi = D.posf[s];
+    const float csf = (float)P.cs;
+    const R f = kernel_factor<DIM, R>(P.skf, hi);
     // kernel gradient correction has solid targets only (create refuses it
     // with fluid real particles)
     const double* A = (TGT != kTgtFluid && P.kgc) ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
@@ -1102,52 +1178,71 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     if (as_on) {
         for (int q = 0; q < 4; ++q) Ri[q] = (ti >= 100) ? (R)D.asR[(int64_t)s * 4 + q] : (R)0.0;
         R G;
-        kernel_wg<DIM, R>(P.skf, (R)P.as_delta, (R)qi.w, f, &wdelta, &G);
+        kernel_wg<DIM, R>(P.skf, (R)P.as_delta, hi, f, &wdelta, &G);
     }
     R acc[3] = {0.0, 0.0, 0.0};
     R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0, drho_r = 0.0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     for (int k = sub; k < n; k += LPT) {
         int j = D.nbr[(int64_t)k * P.ldn + s];
-        double4 qj = D.posh[j];
-        double4 vj = D.vm[j];
-        double4 thj = D.th[j];
         PairGeo<DIM, R> g;
-        pair_geo<DIM, R>(P, qi, qj, f, A, g);
-        const R mj = (R)vj.w;
-        R dv[3] = {(R)(vi.x - vj.x), (R)(vi.y - vj.y), DIM == 3 ? (R)(vi.z - vj.z) : (R)0.0};
+        R vj_r[3], hj, mj, rhoj, prj, cj, Vj;
+        if constexpr (REC32) {
+            const float4 pfj = D.posf[j];
+            const PairRecF rec = D.prf[j];
+            rec_dx<DIM>(pfi, pfj, csf, g.dx);
+            pair_geo_dx<DIM, R>(P, hi, f, A, g);
+            vj_r[0] = rec.vm.x; vj_r[1] = rec.vm.y; vj_r[2] = rec.vm.z;
+            Vj = rec.vm.w;
+            rhoj = rec.th.x; prj = rec.th.y; cj = rec.th.z; hj = rec.th.w;
+            mj = Vj * rhoj;
+        } else {
+            const double4 qj = D.posh[j];
+            const double4 vj = D.vm[j];
+            const double4 thj = D.th[j];
+            pair_geo<DIM, R>(P, qi, qj, f, A, g);
+            vj_r[0] = (R)vj.x; vj_r[1] = (R)vj.y; vj_r[2] = (R)vj.z;
+            Vj = (R)thj.w;
+            rhoj = (R)thj.x; prj = (R)thj.y; cj = (R)thj.z; hj = (R)qj.w;
+            mj = (R)vj.w;
+        }
+        R dv[3] = {vi_r[0] - vj_r[0], vi_r[1] - vj_r[1], DIM == 3 ? vi_r[2] - vj_r[2] : (R)0.0};
         R dvg = dv[0] * g.gw[0] + dv[1] * g.gw[1] + (DIM == 3 ? dv[2] * g.gw[2] : (R)0.0);
         R dxg = g.dx[0] * g.gw[0] + g.dx[1] * g.gw[1] + (DIM == 3 ? g.dx[2] * g.gw[2] : (R)0.0);
         // continuity (density.f90:98-101) and divergences (divergence.f90:17-27)
-        sdiv_v += (R)thj.w * dvg;
-        sdiv_x += (R)thj.w * dxg;
+        sdiv_v += Vj * dvg;
+        sdiv_x += Vj * dxg;
+        // the neighbour's viscous or stress tensor, FP32 copies in mixed precision
+        auto tensor = [&](const double* T64, const float* T32, int q) -> R {
+            if constexpr (REC32) return T32[(int64_t)j * DIM * DIM + q];
+            else return (R)T64[(int64_t)j * DIM * DIM + q];
+        };
         // internal force, SPH = 1 (internalForce.f90:197-225) and SPH = 3,
         // PVRS Riemann (:277-325): fluids only, solids take none
         if (pa_sph != 2) {
             if (fluid) {
-                const R rhoij = (R)1.0 / (R)(thi.x * thj.x);
+                const R rhoij = (R)1.0 / (rhoi * rhoj);
                 const R pi = (R)D.p[s], pj = (R)D.p[j];
                 R a13, w[3];
                 R ex[3] = {0, 0, 0}, v_l = 0, v_r = 0;
-                const R vix[3] = {(R)vi.x, (R)vi.y, (R)vi.z}, vjx[3] = {(R)vj.x, (R)vj.y, (R)vj.z};
                 if (pa_sph == 3) {
                     for (int d = 0; d < DIM; ++d) {
                         ex[d] = -g.dx[d] / g.r;   // (x_j - x_i) / |x_j - x_i|
-                        v_l += vix[d] * ex[d];
-                        v_r += vjx[d] * ex[d];
+                        v_l += vi_r[d] * ex[d];
+                        v_r += vj_r[d] * ex[d];
                     }
                 }
                 if (pa_sph == 1) {
                     a13 = (pi + pj) * rhoij;
                     for (int d = 0; d < DIM; ++d) w[d] = dv[d];
                 } else {
-                    const R Zl = (R)(thi.x * thi.z), Zr = (R)(thj.x * thj.z);
+                    const R Zl = rhoi * ci, Zr = rhoj * cj;
                     const R v_ij = (Zl * v_l + Zr * v_r + (pi - pj)) / (Zl + Zr);
                     const R p_star = (Zl * pj + Zr * pi + Zl * Zr * (v_l - v_r)) / (Zl + Zr);
                     a13 = (R)2 * p_star * rhoij;
                     for (int d = 0; d < DIM; ++d) {
-                        const R vs = v_ij * ex[d] + ((vix[d] + vjx[d]) / (R)2 - ((v_l + v_r) / (R)2) * ex[d]);
-                        w[d] = (R)2 * (vix[d] - vs);
+                        const R vs = v_ij * ex[d] + ((vi_r[d] + vj_r[d]) / (R)2 - ((v_l + v_r) / (R)2) * ex[d]);
+                        w[d] = (R)2 * (vi_r[d] - vs);
                     }
                 }
                 const int tvj = D.meta[j].x;
@@ -1155,10 +1250,8 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
                 R dot = (R)0;
                 for (int d = 0; d < DIM; ++d) {
                     R mv = (R)0.0;   // matmul(eta_i edot_i + eta_j edot_j, dwdx)
-                    if (vj_on) {
-                        const double* Ej = D.Ep + (int64_t)j * DIM * DIM;
-                        for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + (R)Ej[d * DIM + e]) * g.gw[e];
-                    }
+                    if (vj_on)
+                        for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + tensor(D.Ep, D.Epf, d * DIM + e)) * g.gw[e];
                     acc[d] += mj * (-a13 * g.gw[d] + mv * rhoij);
                     dot += w[d] * g.gw[d];
                 }
@@ -1167,43 +1260,38 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
             if (pa_sph == 3) {
                 // density.f90:109-145 with the pressures of the last EOS and
                 // the neighbour's sound speed in both impedances
-                const R cj = (R)thj.z;
-                const R Zl = (R)thi.x * cj, Zr = (R)thj.x * cj;
+                const R Zl = rhoi * cj, Zr = rhoj * cj;
                 const R pi = (R)D.pold[s], pj = (R)D.pold[j];
                 R v_l = 0, v_r = 0, ex[3] = {0, 0, 0};
-                const R vix[3] = {(R)vi.x, (R)vi.y, (R)vi.z}, vjx[3] = {(R)vj.x, (R)vj.y, (R)vj.z};
                 for (int d = 0; d < DIM; ++d) {
                     ex[d] = -g.dx[d] / g.r;
-                    v_l += vix[d] * ex[d];
-                    v_r += vjx[d] * ex[d];
+                    v_l += vi_r[d] * ex[d];
+                    v_r += vj_r[d] * ex[d];
                 }
                 const R v_ij = (Zl * v_l + Zr * v_r + (pi - pj)) / (Zl + Zr);
                 R dot = (R)0;
                 for (int d = 0; d < DIM; ++d) {
-                    const R vs = v_ij * ex[d] + ((vix[d] + vjx[d]) / (R)2 - ((v_l + v_r) / (R)2) * ex[d]);
-                    dot += (vix[d] - vs) * g.gw[d];
+                    const R vs = v_ij * ex[d] + ((vi_r[d] + vj_r[d]) / (R)2 - ((v_l + v_r) / (R)2) * ex[d]);
+                    dot += (vi_r[d] - vs) * g.gw[d];
                 }
-                drho_r += (R)2 * (R)thi.x * mj / (R)thj.x * dot;
+                drho_r += (R)2 * rhoi * mj / rhoj * dot;
             }
         } else {
         // internal force, SPH = 2 (internalForce.f90:228-274)
-        R aux = (R)(thi.y + thj.y);
+        R aux = pri + prj;
         if (fluid) {
             const int tvj = D.meta[j].x;
             const bool vj_on = visc && !(P.visc_skip_solid && (tvj > 100 || tvj < -100));
             for (int d = 0; d < DIM; ++d) {
                 R mv = (R)0.0;
-                if (vj_on) {
-                    const double* Ej = D.Ep + (int64_t)j * DIM * DIM;
-                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + (R)Ej[d * DIM + e]) * g.gw[e];
-                }
+                if (vj_on)
+                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + tensor(D.Ep, D.Epf, d * DIM + e)) * g.gw[e];
                 acc[d] += mj * (-aux * g.gw[d] + mv);
             }
         } else {
-            const double* Sj = D.Sp + (int64_t)j * DIM * DIM;
             for (int d = 0; d < DIM; ++d) {
                 R sv = (R)0.0;
-                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + (R)Sj[d * DIM + e]) * g.gw[e];
+                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + tensor(D.Sp, D.Spf, d * DIM + e)) * g.gw[e];
                 acc[d] += mj * sv;
             }
         }
@@ -1219,13 +1307,13 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
             }
             if (ti != tj && ti > 0 && tj > 0) {
                 if (ti < 100 && tj < 100) {
-                    R pe = (R)(qi.w + qj.w) / ((R)2 * g.r);
+                    R pe = (hi + hj) / ((R)2 * g.r);
                     if (pe >= (R)1) {
                         R fp = (R)P.pen_factor * (pow(pe, (R)P.pen_n1) - pow(pe, (R)P.pen_n2)) / (g.r * g.r);
                         for (int d = 0; d < DIM; ++d) acc[d] += fp * g.dx[d];
                     }
                 } else if (!(ti > 100 && tj > 100)) {
-                    R et = g.r / (R)0.75 / ((R)(qi.w + qj.w) * (R)0.5);
+                    R et = g.r / (R)0.75 / ((hi + hj) * (R)0.5);
                     R chi = (R)1 - g.r / (R)P.couple_delta;
                     if (chi < (R)0 || chi > (R)1) chi = (R)0;
                     R fc = (R)0;
@@ -1242,9 +1330,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         if (P.arti_visc) {
             R xv = g.dx[0] * dv[0] + g.dx[1] * dv[1] + (DIM == 3 ? g.dx[2] * dv[2] : (R)0.0);
             if (xv < (R)0.0) {
-                R hij = (R)(0.5 * (qi.w + qj.w));
-                R rhoij = (R)(0.5 * (thi.x + thj.x));
-                R cij = (R)(0.5 * (thi.z + thj.z));
+                R hij = (R)0.5 * (hi + hj);
+                R rhoij = (R)0.5 * (rhoi + rhoj);
+                R cij = (R)0.5 * (ci + cj);
                 R ph = (R)P.av_psi * hij;
                 R phi = hij * xv / (g.r2 + ph * ph);
                 R PI_ij = ((R)P.av_beta * phi - (R)P.av_alpha * cij) * phi / rhoij;
@@ -1300,12 +1388,6 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-
-template <int DIM>
-__device__ __forceinline__ void unpack_cell(uint32_t pk, int& x, int& y, int& z) {
-    if (DIM == 3) { x = (int)(pk & 2047u); y = (int)((pk >> 11) & 2047u); z = (int)((pk >> 22) & 511u); }
-    else { x = (int)(pk & 65535u); y = (int)((pk >> 16) & 32767u); z = 0; }
-}
 
 template <int DIM, bool MIXED, bool PAIR_EXT>
 __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
@@ -2335,9 +2417,15 @@ static int alloc_state(sphgpu_ctx* c) {
     rc |= dalloc(c, &D.perm, N); rc |= dalloc(c, &D.ncnt, N);
     rc |= dalloc(c, &D.cpos, N);
     rc |= dalloc(c, &D.vcnt, N); rc |= dalloc(c, &D.ecnt, N);
-    // FP32 position records: the mixed fast path's pair inputs, and the
-    // pre-test of every candidate path
-    if (c->P.mixed || c->use_verlet) rc |= dalloc(c, &D.posf, N);
+    // FP32 position records: the mixed fast path's pair inputs, the pre-test
+    // of every candidate path and the mixed general path's pair geometry
+    if (c->P.mixed || c->use_verlet || c->P.pair32) rc |= dalloc(c, &D.posf, N);
+    // the mixed general path gathers FP32 pair records and tensors
+    if (c->P.pair32) {
+        rc |= dalloc(c, &D.prf, N);
+        if (c->cfg.viscosity || c->P.has_solid) rc |= dalloc(c, &D.Epf, T);
+        if (c->P.has_solid) rc |= dalloc(c, &D.Spf, T);
+    }
     rc |= dalloc(c, &D.vflag, 2);
     bool need_tensor = c->cfg.viscosity || c->P.has_solid;
     if (need_tensor) rc |= dalloc(c, &D.Ep, T);
