@@ -356,7 +356,6 @@ extern "C" int sphgpu_batch_add(sphgpu_batch* b, const sphgpu_config* cfg, int64
     P.maxcnt = b->d_maxcnt + 4 * row;
     P.partial = b->D.partial + 6 * (s0 / kBlock);
     P.mat = b->d_mat + SPHGPU_MAX_MATERIALS * row;
-    P.ldn = b->slot_cap;
     P.verlet_cap = b->verlet_cap;
     P.nbr_cap = 0;
     Material tab[SPHGPU_MAX_MATERIALS];

@@ -11,22 +11,6 @@
 // the reference's arithmetic (asymmetricSearch.f90:227-237) and window
 // (:209-217).
 
-// Candidate-list layout: each group of kVlistLanes consecutive slots (one
-// wave) owns a contiguous block of cap rows of kVlistLanes entries; entry k of
-// slot s is at row k of its group's block. A wave's load of entry k is one
-// contiguous 256-byte row, and its whole list is one contiguous stream, so
-// HBM reads it page by page instead of one row per DRAM page.
-constexpr int kVlistLanes = 64;
-
-__host__ __device__ __forceinline__ int64_t vlist_at(int s, int k, int cap) {
-    return ((int64_t)(s / kVlistLanes) * cap + k) * kVlistLanes + s % kVlistLanes;
-}
-
-// entries of a candidate list for `slots` slots with `cap` rows each
-inline int64_t vlist_entries(int64_t slots, int cap) {
-    return (slots + kVlistLanes - 1) / kVlistLanes * kVlistLanes * cap;
-}
-
 // Predict + EOS + pair inputs for every particle, the reference cell (for the
 // window check), and the largest displacement since the last list build.
 template <int DIM, bool MIXED, bool BATCH>

@@ -51,10 +51,10 @@ __global__ __launch_bounds__(kBlock) void k_point_gen(Params P, Dev D, int mode)
 
 // Exact neighbour list of this step from the candidate list: the reference's
 // distance test and window on every candidate, kept in candidate order (the
-// order of the cell sweep at the build), nbr[k * ldn + s] as the pair kernels
-// read it. The distance is tested in FP32 from the cell-offset records first;
-// only a candidate within the FP32 error bound of the cutoff (or off the grid)
-// loads its FP64 position for the reference's test. A particle whose
+// order of the cell sweep at the build), in the wave-blocked layout the pair
+// kernels read. The distance is tested in FP32 from the cell-offset records
+// first; only a candidate within the FP32 error bound of the cutoff (or off
+// the grid) loads its FP64 position for the reference's test. A particle whose
 // candidates outgrew its list walks its cell window of the last sort instead.
 // Closes the rebuild gate of the step: every gated kernel ran before this
 // launch.
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
             for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
             if (!inwin) return;
         }
-        if (cnt < cap) D.nbr[(int64_t)cnt * P.ldn + s] = j;
+        if (cnt < cap) D.nbr[vlist_at(s, cnt, cap)] = j;
         ++cnt;
     };
     const int nc = D.vcnt[s];

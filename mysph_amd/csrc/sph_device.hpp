@@ -56,7 +56,6 @@ struct Params {
     int win;            // sweep half-width in cells
     int64_t ncell;      // nc[0]*nc[1]*nc[2]; cell index ncell holds State == -1 particles
     int nbr_cap;        // neighbour-list slots per particle
-    int64_t ldn;        // leading dimension of the neighbour list (particle capacity)
     // fluid fast path: candidate (Verlet) list within cutoff + skin, rebuilt
     // when some particle has moved skin/2 since the last build
     double skin;
@@ -140,6 +139,22 @@ __device__ __forceinline__ void vote_rebuild(const Params& P, bool moved) {
         if (P.vany != P.vflag && __atomic_load_n(&P.vany[1], __ATOMIC_RELAXED) == 0ull)
             atomicOr(&P.vany[1], 1ull);
     }
+}
+
+// Neighbour-list layout (candidate lists and exact lists): each group of
+// kVlistLanes consecutive slots (one wave) owns a contiguous block of cap rows of kVlistLanes entries; entry k of
+// slot s is at row k of its group's block. A wave's load of entry k is one
+// contiguous 256-byte row, and its whole list is one contiguous stream, so
+// HBM reads it page by page instead of one row per DRAM page.
+constexpr int kVlistLanes = 64;
+
+__host__ __device__ __forceinline__ int64_t vlist_at(int s, int k, int cap) {
+    return ((int64_t)(s / kVlistLanes) * cap + k) * kVlistLanes + s % kVlistLanes;
+}
+
+// entries of a candidate list for `slots` slots with `cap` rows each
+inline int64_t vlist_entries(int64_t slots, int cap) {
+    return (slots + kVlistLanes - 1) / kVlistLanes * kVlistLanes * cap;
 }
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {

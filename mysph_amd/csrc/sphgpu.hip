@@ -774,7 +774,7 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
             for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
             if (!inwin) return;
         }
-        if (cnt < cap) D.nbr[(int64_t)cnt * P.ldn + s] = j;
+        if (cnt < cap) D.nbr[vlist_at(s, cnt, cap)] = j;
         ++cnt;
     };
     // rows of cells along x are contiguous in cell order: one range per (y, z);
@@ -896,7 +896,7 @@ __global__ __launch_bounds__(kBlock) void k_kgc(Params P, Dev D) {
         int n = min(D.ncnt[s], P.nbr_cap);
         int ai = mt.x < 0 ? -mt.x : mt.x;
         for (int k = sub; k < n; k += LPT) {
-            int j = D.nbr[(int64_t)k * P.ldn + s];
+            int j = D.nbr[vlist_at(s, k, P.nbr_cap)];
             int tj = D.meta[j].x;
             int aj = tj < 0 ? -tj : tj;
             if ((ai - 100) * (aj - 100) < 0) continue;
@@ -934,7 +934,7 @@ __global__ __launch_bounds__(kBlock) void k_sumdens(Params P, Dev D) {
     if (norm && sub == 0) wi = mi / (R)D.rho[s] * w0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     for (int k = sub; k < n; k += LPT) {
-        int j = D.nbr[(int64_t)k * P.ldn + s];
+        int j = D.nbr[vlist_at(s, k, P.nbr_cap)];
         double4 qj = D.posh[j];
         PairGeo<DIM, R> g;
         pair_geo<DIM, R>(P, qi, qj, f, nullptr, g);
@@ -986,7 +986,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
         const float csf = (float)P.cs;
         int n = min(D.ncnt[s], P.nbr_cap);
         for (int k = sub; k < n; k += LPT) {
-            int j = D.nbr[(int64_t)k * P.ldn + s];
+            int j = D.nbr[vlist_at(s, k, P.nbr_cap)];
             PairGeo<DIM, R> g;
             R V, dvn[3];
             if constexpr (REC32) {
@@ -1184,7 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0, drho_r = 0.0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
     for (int k = sub; k < n; k += LPT) {
-        int j = D.nbr[(int64_t)k * P.ldn + s];
+        int j = D.nbr[vlist_at(s, k, P.nbr_cap)];
         PairGeo<DIM, R> g;
         R vj_r[3], hj, mj, rhoj, prj, cj, Vj;
         if constexpr (REC32) {
@@ -1427,11 +1427,11 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
     const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, psi2 = (R)(P.av_psi * P.av_psi);
     R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
     const int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
-    const int* __restrict__ nl = D.nbr + s;
+    const int* __restrict__ nl = D.nbr + vlist_at(s, 0, P.nbr_cap);
     int jn = n > 0 ? nl[0] : 0;
     for (int k = 0; k < n; ++k) {
         const int j = jn;
-        if (k + 1 < n) jn = nl[(int64_t)(k + 1) * P.ldn];
+        if (k + 1 < n) jn = nl[(int64_t)(k + 1) * kVlistLanes];
         R4 vj, thj;
         R dx, dy, dz, hj, Vj;
         if (MIXED) {
@@ -1968,19 +1968,42 @@ static bool fluid_fast_path(const sphgpu_ctx* c) { return fast_path_ok(c->cfg, c
 // Our search grid: cells of cutoff/win (slightly enlarged) over the upload
 // bounding box plus a margin; keys outside are clamped to the border cells,
 // which keeps every neighbour within the +-win sweep.
-// Candidate-list tuning of a run: cells per cutoff and skin (fraction of the
-// cutoff). Neither changes which pairs interact; both change the speed.
+// Tuning of a run: cells per cutoff, the candidate skin (fraction of the
+// cutoff) and the edges of the slot-order tiles in cells (y and z, and x);
+// 0 takes the path's default. None changes which pairs interact; all change
+// the speed.
 struct Tuning {
     int win;
     double skin_frac;
+    int tile;
+    int tile_x;
 };
+
+// Default skins: the fluid fast path's fused pass tests every candidate with
+// full pair arithmetic and is rebuilt rarely; the general path's filter is a
+// pass of its own whose cost is one gather per candidate, so a thinner skin
+// (more rebuilds, fewer candidates) is faster there.
+constexpr double kSkinFast = 0.15, kSkinGeneral = 0.07;
+// Default slot-order tiles in 3D (2D keeps whole rows and planes: its
+// neighbours are a few rows apart). The fast path's fused pass gains most
+// when the lanes of a wave are consecutive cells of one x-row, whose k-th
+// candidates sit side by side: its tiles keep whole x-rows and bound y and z
+// (8 cells). The general path's pair passes gather more bytes per pair and
+// gain more from cubic tiles (64 cells), whose neighbour records stay in an
+// XCD's L2. Both measured on the benchmark dam break and the elastic gate.
+constexpr int kTileFast3 = 8, kTileGeneral3 = 64;
+constexpr int kTileWhole = 1 << 30;
 
 static Tuning env_tuning() {
     const char* wenv = getenv("SPHGPU_WIN");
     const char* senv = getenv("SPHGPU_SKIN");
+    const char* tenv = getenv("SPHGPU_TILE");
     Tuning t;
     t.win = (wenv && wenv[0] >= '1' && wenv[0] <= '4') ? wenv[0] - '0' : kWinDefault;
-    t.skin_frac = senv ? atof(senv) : 0.15;
+    t.skin_frac = senv ? atof(senv) : 0.0;
+    t.tile = tenv ? std::max(0, atoi(tenv)) : 0;
+    const char* txenv = getenv("SPHGPU_TILE_X");
+    t.tile_x = txenv ? std::max(0, atoi(txenv)) : 0;
     return t;
 }
 
@@ -2487,7 +2510,7 @@ static int alloc_cells(sphgpu_ctx* c) {
 }
 
 static int alloc_nbr(sphgpu_ctx* c, int cap) {
-    int64_t need = (int64_t)cap * c->capacity;
+    int64_t need = vlist_entries(c->capacity, cap);
     if (need <= c->nbr_alloc) return 0;
     if (dalloc(c, &c->D.nbr, (size_t)need)) return SPHGPU_E_HIP;
     c->nbr_alloc = need;
@@ -2521,7 +2544,6 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     if (c->P.sle && c->P.hmax_upload > 0)
         cap = (int)std::ceil(cap * std::pow(c->P.hmax / c->P.hmax_upload, c->dim));
     c->P.nbr_cap = cap;
-    c->P.ldn = c->capacity;
     e = alloc_nbr(c, cap);
     if (e) return e;
     if (fluid_fast_path(c) && c->use_verlet) {
@@ -2643,9 +2665,29 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     if (P.sle == 1)   // hsml.f90:22-25, with the uploaded density
         for (int64_t i = 0; i < n_real; ++i)
             if (rho[i] > 0) h_next_max = std::max(h_next_max, 2.0 * std::pow(mass[i] / rho[i], 1.0 / dim));
+    if (!(tune.skin_frac > 0)) tune.skin_frac = hr.fast ? kSkinFast : kSkinGeneral;
     plan_grid_params(P, dim, use_verlet, n, x, h, tune, h_next_max);
-    // slots by cell key, ascending particle index within a cell: a stable
-    // counting sort over the P.ncell (< 2^31) cells
+    // Slot order: the grid's cells in tiles of te[0] x te[1] (x te[2]) cells,
+    // tiles in lexicographic order and cells lexicographic within a tile,
+    // ascending particle index within a cell, by a stable counting sort over
+    // the tiled cell keys. A wave, a block and an XCD's range of blocks then
+    // cover compact regions, whose neighbours' records stay in that XCD's
+    // L2; lexicographic order (one tile) spreads a particle's neighbours over
+    // several planes of cells.
+    const int T = tune.tile > 0 ? tune.tile : dim == 3 ? (hr.fast ? kTileFast3 : kTileGeneral3) : kTileWhole;
+    const int TX = tune.tile_x > 0 ? tune.tile_x : (hr.fast || dim == 2) ? kTileWhole : T;
+    int64_t te[3] = {1, 1, 1}, tn[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) {
+        te[d] = std::min<int64_t>(d == 0 ? TX : T, P.nc[d]);
+        tn[d] = (P.nc[d] + te[d] - 1) / te[d];
+    }
+    int64_t nkeys = tn[0] * tn[1] * tn[2] * te[0] * te[1] * te[2];
+    if (nkeys > 2 * P.ncell + 4096 || nkeys >= (1LL << 31)) {
+        // tiles that round the grid up this much: one tile (lexicographic)
+        for (int d = 0; d < 3; ++d) { te[d] = P.nc[d]; tn[d] = 1; }
+        nkeys = P.ncell;
+    }
+    const int64_t tcells = te[0] * te[1] * te[2];
     std::vector<uint32_t>& key = hr.sort_key;
     std::vector<uint32_t>& start = hr.sort_start;
     std::vector<int64_t>& order = hr.order;
@@ -2658,22 +2700,24 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
                 t = std::min(std::max(t, 0.0), (double)(P.nc[d] - 1));
                 cc[d] = (int64_t)t;
             }
-            key[i] = (uint32_t)(cc[0] + P.nc[0] * (cc[1] + (int64_t)P.nc[1] * cc[2]));
+            const int64_t tile = (cc[2] / te[2] * tn[1] + cc[1] / te[1]) * tn[0] + cc[0] / te[0];
+            const int64_t local = (cc[2] % te[2] * te[1] + cc[1] % te[1]) * te[0] + cc[0] % te[0];
+            key[i] = (uint32_t)(tile * tcells + local);
         }
     });
-    // each thread owns a range of cells and scans every particle in index
-    // order: the counts and placements of its cells are its own, and a cell
+    // each thread owns a range of keys and scans every particle in index
+    // order: the counts and placements of its keys are its own, and a cell
     // keeps ascending indices
-    start.assign((size_t)P.ncell + 1, 0u);
-    host_parallel(P.ncell, [&](int64_t klo, int64_t khi) {
+    start.assign((size_t)nkeys + 1, 0u);
+    host_parallel(nkeys, [&](int64_t klo, int64_t khi) {
         for (int64_t i = 0; i < n; ++i) {
             const int64_t k = key[i];
             if (k >= klo && k < khi) ++start[k + 1];
         }
     });
-    for (int64_t k = 0; k < P.ncell; ++k) start[k + 1] += start[k];
+    for (int64_t k = 0; k < nkeys; ++k) start[k + 1] += start[k];
     order.resize(n);
-    host_parallel(P.ncell, [&](int64_t klo, int64_t khi) {
+    host_parallel(nkeys, [&](int64_t klo, int64_t khi) {
         for (int64_t i = 0; i < n; ++i) {
             const int64_t k = key[i];
             if (k >= klo && k < khi) order[start[k]++] = i;
@@ -3148,15 +3192,15 @@ extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr
     }
     if (needed) *needed = offsets[nr];
     if (!nbr || offsets[nr] > cap) return fail(c, SPHGPU_E_PAIR_CAPACITY, "pair buffer too small");
-    const int64_t ld = c->P.ldn, lcap = c->P.nbr_cap;
-    std::vector<int> all((size_t)(ld * lcap));
+    const int64_t lcap = c->P.nbr_cap;
+    std::vector<int> all((size_t)vlist_entries(n, (int)lcap));
     HIPCHK(c, hipMemcpyAsync(all.data(), c->D.nbr, all.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < nr; ++i) {
         int64_t s = slot_of[i];
         int64_t o = offsets[i];
         int k = (int)(offsets[i + 1] - offsets[i]);
-        for (int q = 0; q < k; ++q) nbr[o + q] = meta[all[(size_t)(q * ld + s)]].w;
+        for (int q = 0; q < k; ++q) nbr[o + q] = meta[all[(size_t)vlist_at((int)s, q, (int)lcap)]].w;
         std::sort(nbr + o, nbr + o + k);
     }
     return 0;
