@@ -160,3 +160,33 @@ __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
     if (cnt > cap) atomicOr(P.err, kErrPairCap);
     if (cnt > __atomic_load_n(P.maxcnt, __ATOMIC_RELAXED)) atomicMax(P.maxcnt, cnt);
 }
+
+// The step boundary of the general path: the correct of step n
+// (timeIntegration.f90:178-252) fused with the predict + EOS of step n + 1
+// (:137-170) for real particles, their FP32 records, the displacement vote,
+// and the bounding-box partials of the new positions for the grid of step
+// n + 1. Dummies are static: their EOS and records were written by
+// k_point_gen and do not change. The reference cells follow once the new grid
+// is known (k_refcell).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_correct_point_gen(Params P, Dev D, int first) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool moved = false;
+    if (s < P.n) {
+        const int4 mt = D.meta[s];
+        double4 q = D.posh[s];
+        if (mt.w < P.n_real) {
+            correct_one<DIM, false>(P, D, s, mt, first != 0, q);
+            predict_one<DIM>(P, D, s, mt, false);
+            if (P.pa_sph == 3) D.pold[s] = D.p[s];
+            if (!P.summation_density) eos_th(P, D, s, mt);
+            D.posf[s] = cell_offset_record(P, q);
+            if (P.pair32) write_rec32(D, s);
+            moved = moved_half_skin<false>(P, D, s, q);
+        }
+        mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
+    }
+    vote_rebuild(P, moved);
+    bbox_block_reduce(mn, mx, P.partial, blockIdx.x);
+}

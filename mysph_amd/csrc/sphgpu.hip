@@ -962,7 +962,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
 
 // Strain-rate (and solid rotation-rate) sums, internalForce.f90:48-100, and the
 // fluid viscous heating tdsdt (:108-112). T(d,e) = sum_j V_j (v_j - v_i)_d dW_e.
-template <int DIM, typename R, int LPT>
+template <int DIM, typename R, int LPT, bool KGC>
 __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
     int sub;
     const int s = target_slot<LPT>(P, nullptr, 0, sub);
@@ -980,13 +980,17 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
         double4 qi = D.posh[s];
         double4 vi = D.vm[s];
         R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
-        const double* A = P.kgc ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
+        // KGC: the run corrects kernel gradients (compiled out otherwise)
+        const double* A = KGC ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
         float4 pfi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (REC32) pfi = D.posf[s];
         const float csf = (float)P.cs;
         int n = min(D.ncnt[s], P.nbr_cap);
+        // the next index is in flight while a pair is summed
+        int jn = sub < n ? D.nbr[vlist_at(s, sub, P.nbr_cap)] : 0;
         for (int k = sub; k < n; k += LPT) {
-            int j = D.nbr[vlist_at(s, k, P.nbr_cap)];
+            const int j = jn;
+            if (k + LPT < n) jn = D.nbr[vlist_at(s, k + LPT, P.nbr_cap)];
             PairGeo<DIM, R> g;
             R V, dvn[3];
             if constexpr (REC32) {
@@ -1002,7 +1006,7 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
                 double4 qj = D.posh[j];
                 double4 vj = D.vm[j];
                 pair_geo<DIM, R>(P, qi, qj, f, A, g);
-                V = (R)(vj.w / D.th[j].x);
+                V = (R)D.th[j].w;   // m / rho of this step's EOS
                 dvn[0] = (R)(vj.x - vi.x);
                 dvn[1] = (R)(vj.y - vi.y);
                 dvn[2] = (R)(vj.z - vi.z);
@@ -1183,8 +1187,24 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     R acc[3] = {0.0, 0.0, 0.0};
     R de_in = 0.0, de_av = 0.0, sdiv_v = 0.0, sdiv_x = 0.0, drho_r = 0.0;
     int n = (mt.y == -1) ? 0 : min(D.ncnt[s], P.nbr_cap);
+    // the neighbour type is read by the viscous, wall/coupling and artificial
+    // stress terms; each pair's loads issue together, and the next index is
+    // in flight while a pair is summed
+    const bool need_type = visc || P.external_force || as_on;
+    const bool need_t = visc || !fluid;
+    int jn = sub < n ? D.nbr[vlist_at(s, sub, P.nbr_cap)] : 0;
     for (int k = sub; k < n; k += LPT) {
-        int j = D.nbr[vlist_at(s, k, P.nbr_cap)];
+        const int j = jn;
+        if (k + LPT < n) jn = D.nbr[vlist_at(s, k + LPT, P.nbr_cap)];
+        const int tj = need_type ? D.meta[j].x : 0;
+        // the neighbour's viscous (fluid target) or stress (solid target)
+        // tensor, FP32 copies in mixed precision
+        R Tj[DIM * DIM];
+        if (need_t)
+            for (int q = 0; q < DIM * DIM; ++q) {
+                if constexpr (REC32) Tj[q] = (fluid ? D.Epf : D.Spf)[(int64_t)j * DIM * DIM + q];
+                else Tj[q] = (R)(fluid ? D.Ep : D.Sp)[(int64_t)j * DIM * DIM + q];
+            }
         PairGeo<DIM, R> g;
         R vj_r[3], hj, mj, rhoj, prj, cj, Vj;
         if constexpr (REC32) {
@@ -1212,11 +1232,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         // continuity (density.f90:98-101) and divergences (divergence.f90:17-27)
         sdiv_v += Vj * dvg;
         sdiv_x += Vj * dxg;
-        // the neighbour's viscous or stress tensor, FP32 copies in mixed precision
-        auto tensor = [&](const double* T64, const float* T32, int q) -> R {
-            if constexpr (REC32) return T32[(int64_t)j * DIM * DIM + q];
-            else return (R)T64[(int64_t)j * DIM * DIM + q];
-        };
         // internal force, SPH = 1 (internalForce.f90:197-225) and SPH = 3,
         // PVRS Riemann (:277-325): fluids only, solids take none
         if (pa_sph != 2) {
@@ -1245,13 +1260,12 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
                         w[d] = (R)2 * (vi_r[d] - vs);
                     }
                 }
-                const int tvj = D.meta[j].x;
-                const bool vj_on = visc && !(P.visc_skip_solid && (tvj > 100 || tvj < -100));
+                const bool vj_on = visc && !(P.visc_skip_solid && (tj > 100 || tj < -100));
                 R dot = (R)0;
                 for (int d = 0; d < DIM; ++d) {
                     R mv = (R)0.0;   // matmul(eta_i edot_i + eta_j edot_j, dwdx)
                     if (vj_on)
-                        for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + tensor(D.Ep, D.Epf, d * DIM + e)) * g.gw[e];
+                        for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + Tj[d * DIM + e]) * g.gw[e];
                     acc[d] += mj * (-a13 * g.gw[d] + mv * rhoij);
                     dot += w[d] * g.gw[d];
                 }
@@ -1280,18 +1294,17 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         // internal force, SPH = 2 (internalForce.f90:228-274)
         R aux = pri + prj;
         if (fluid) {
-            const int tvj = D.meta[j].x;
-            const bool vj_on = visc && !(P.visc_skip_solid && (tvj > 100 || tvj < -100));
+            const bool vj_on = visc && !(P.visc_skip_solid && (tj > 100 || tj < -100));
             for (int d = 0; d < DIM; ++d) {
                 R mv = (R)0.0;
                 if (vj_on)
-                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + tensor(D.Ep, D.Epf, d * DIM + e)) * g.gw[e];
+                    for (int e = 0; e < DIM; ++e) mv += (Ei[d * DIM + e] + Tj[d * DIM + e]) * g.gw[e];
                 acc[d] += mj * (-aux * g.gw[d] + mv);
             }
         } else {
             for (int d = 0; d < DIM; ++d) {
                 R sv = (R)0.0;
-                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + tensor(D.Sp, D.Spf, d * DIM + e)) * g.gw[e];
+                for (int e = 0; e < DIM; ++e) sv += (Si[d * DIM + e] + Tj[d * DIM + e]) * g.gw[e];
                 acc[d] += mj * sv;
             }
         }
@@ -1299,7 +1312,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         }
         // external pair forces (externalForce.f90:103-174)
         if (P.external_force) {
-            const int tj = D.meta[j].x;
             if (ti > 0 && tj < 0 && (double)g.r < P.lj_r0) {
                 R rr = (R)P.lj_r0 / g.r;
                 R fl = (R)P.lj_factor * (pow(rr, (R)P.lj_p1) - pow(rr, (R)P.lj_p2)) / (g.r * g.r);
@@ -1342,7 +1354,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         }
         // artificial stress (artificialTerm.f90:72-93)
         if (as_on) {
-            const int tj = D.meta[j].x;
             if (tj >= 100) {
                 R F = pow(g.w / wdelta, (R)P.as_exponent);
                 const double* Rj = D.asR + (int64_t)j * 4;
@@ -1534,6 +1545,68 @@ __global__ __launch_bounds__(kBlock) void k_force_fluid(Params P, Dev D) {
 
 // Leap-frog correct (timeIntegration.f90:178-252) and bounding-box partials
 // of the new positions for the next step's grid.
+// Leap-frog correct of one real particle (timeIntegration.f90:178-252): the
+// velocity, density and energy from their values at the last predict, the
+// position, and the deviatoric stress of a solid (J2 cap). q: the particle's
+// position record, updated.
+template <int DIM, bool MIXED>
+__device__ __forceinline__ void correct_one(const Params& P, const Dev& D, int s, int4 mt, bool first,
+                                            double4& q) {
+    const double dt = P.dt;
+    const double fac = first ? dt / 2 : dt;
+    double4 vp = D.vprev[s];
+    double dvx, dvy, dvz, dee;
+    bool finite_rho;
+    if (MIXED) {
+        float4 dv = D.dvf[s];
+        dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
+        float dr = D.drho_prev[s] + (float)fac * dv.w;
+        D.drho[s] = dr;
+        finite_rho = isfinite(dr);
+    } else {
+        double4 dv = D.dv[s];
+        dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
+        if (!P.summation_density) D.rho[s] = vp.w + fac * dv.w;
+        finite_rho = isfinite(D.rho[s]);
+    }
+    if (MIXED) {
+        float e = D.eprevf[s] + (float)fac * (float)dee;
+        if (e < 0) e = 0;
+        D.ef[s] = e;
+    } else {
+        double e = D.eprev[s] + fac * dee;
+        if (e < 0) e = 0;
+        D.e[s] = e;
+    }
+    double4 v = D.vm[s];
+    v.x = vp.x + fac * dvx;
+    v.y = vp.y + fac * dvy;
+    if (DIM == 3) v.z = vp.z + fac * dvz;
+    D.vm[s] = v;
+    add_disp<DIM, MIXED>(D, s, first, dt, v);
+    q.x = q.x + dt * v.x;
+    q.y = q.y + dt * v.y;
+    if (DIM == 3) q.z = q.z + dt * v.z;
+    D.posh[s] = q;
+    if (!MIXED && P.has_solid && mt.x > 100) {
+        const Material* m = find_mat(P, mt.x);
+        if (m) {
+            double* S = D.shear + (int64_t)s * DIM * DIM;
+            const double* Sp = D.shear_prev + (int64_t)s * DIM * DIM;
+            const double* dS = D.dsdt + (int64_t)s * DIM * DIM;
+            for (int k = 0; k < DIM * DIM; ++k) S[k] = (first ? S[k] : Sp[k]) + fac * dS[k];
+            if (m->sigma_y > 0) {
+                double J2 = 0.0;
+                for (int k = 0; k < DIM * DIM; ++k) J2 += S[k] * S[k];
+                double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
+                for (int k = 0; k < DIM * DIM; ++k) S[k] = S[k] * sc;
+            }
+        }
+    }
+    if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
+        atomicOr(P.err, kErrNonFinite);
+}
+
 template <int DIM, bool MIXED, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_correct(Params P0, Dev D, int first_arg, RunMap M, int kstep) {
     int lb;
@@ -1545,61 +1618,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(Params P0, Dev D, int first_
     if (l < P.n) {
         int4 mt = D.meta[s];
         double4 q = D.posh[s];
-        if (mt.w < P.n_real) {
-            const double dt = P.dt;
-            const double fac = first ? dt / 2 : dt;
-            double4 vp = D.vprev[s];
-            double dvx, dvy, dvz, dee;
-            bool finite_rho;
-            if (MIXED) {
-                float4 dv = D.dvf[s];
-                dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
-                float dr = D.drho_prev[s] + (float)fac * dv.w;
-                D.drho[s] = dr;
-                finite_rho = isfinite(dr);
-            } else {
-                double4 dv = D.dv[s];
-                dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
-                if (!P.summation_density) D.rho[s] = vp.w + fac * dv.w;
-                finite_rho = isfinite(D.rho[s]);
-            }
-            if (MIXED) {
-                float e = D.eprevf[s] + (float)fac * (float)dee;
-                if (e < 0) e = 0;
-                D.ef[s] = e;
-            } else {
-                double e = D.eprev[s] + fac * dee;
-                if (e < 0) e = 0;
-                D.e[s] = e;
-            }
-            double4 v = D.vm[s];
-            v.x = vp.x + fac * dvx;
-            v.y = vp.y + fac * dvy;
-            if (DIM == 3) v.z = vp.z + fac * dvz;
-            D.vm[s] = v;
-            add_disp<DIM, MIXED>(D, s, first, dt, v);
-            q.x = q.x + dt * v.x;
-            q.y = q.y + dt * v.y;
-            if (DIM == 3) q.z = q.z + dt * v.z;
-            D.posh[s] = q;
-            if (!MIXED && P.has_solid && mt.x > 100) {
-                const Material* m = find_mat(P, mt.x);
-                if (m) {
-                    double* S = D.shear + (int64_t)s * DIM * DIM;
-                    const double* Sp = D.shear_prev + (int64_t)s * DIM * DIM;
-                    const double* dS = D.dsdt + (int64_t)s * DIM * DIM;
-                    for (int k = 0; k < DIM * DIM; ++k) S[k] = (first ? S[k] : Sp[k]) + fac * dS[k];
-                    if (m->sigma_y > 0) {
-                        double J2 = 0.0;
-                        for (int k = 0; k < DIM * DIM; ++k) J2 += S[k] * S[k];
-                        double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
-                        for (int k = 0; k < DIM * DIM; ++k) S[k] = S[k] * sc;
-                    }
-                }
-            }
-            if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
-                atomicOr(P.err, kErrNonFinite);
-        }
+        if (mt.w < P.n_real) correct_one<DIM, MIXED>(P, D, s, mt, first, q);
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
     }
     bbox_block_reduce(mn, mx, P.partial, lb);
@@ -2259,8 +2278,10 @@ static void launch_pair_passes(sphgpu_ctx* c) {
         hipLaunchKernelGGL((k_sumdens<DIM, R, LPT>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
         hipLaunchKernelGGL(k_apply_sumdens, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     }
-    if (P.viscosity || P.has_solid)
-        hipLaunchKernelGGL((k_strain<DIM, R, LPT>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+    if ((P.viscosity || P.has_solid) && P.kgc)
+        hipLaunchKernelGGL((k_strain<DIM, R, LPT, true>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (P.viscosity || P.has_solid)
+        hipLaunchKernelGGL((k_strain<DIM, R, LPT, false>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
 }
 
 // The general force pass: one launch per target class present (fluid, solid),
@@ -2300,39 +2321,13 @@ static void launch_force_general(sphgpu_ctx* c) {
     else launch_force_classes<DIM, R, 0>(c);
 }
 
-// The general path's search with a candidate list: reference grid, predict +
-// EOS + reference cells + displacement vote, the rebuild (cell sort and
-// candidate list) when the gate is open, and this step's exact list from the
-// candidates. count_only: size the candidate list (no exact list).
+// Pair passes and force pass of one evaluation of the derivatives, after the
+// search of any path but the fluid fast path's.
 template <int DIM>
-static void launch_gen_search(sphgpu_ctx* c, int mode, int count_only) {
-    Params& P = c->P;
-    const int nb = (int)((P.n + kBlock - 1) / kBlock);
-    const Gate G = fast_gate(c);
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
-    hipLaunchKernelGGL(k_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode);
-    launch_sort<DIM>(c, G, 0);
-    hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only,
-                       RunMap{});
-    if (!count_only) hipLaunchKernelGGL(k_filter<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
-}
-
-template <int DIM>
-static void launch_derivs(sphgpu_ctx* c, int mode) {
+static void launch_physics(sphgpu_ctx* c) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const bool fast = fluid_fast_path(c);
-    if (fast && c->use_verlet) {
-        launch_fast<DIM>(c, mode);
-        return;
-    }
-    if (c->use_verlet) {
-        launch_gen_search<DIM>(c, mode, 0);
-    } else {
-        hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
-        launch_sort<DIM>(c);
-        launch_point_build<DIM>(c, mode);
-    }
     if (!fast) {
         // pair32: the pair sums of the general path in FP32 (mixed precision)
         if (P.n_real <= kSmallRun) {
@@ -2362,6 +2357,49 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
     if (P.sle) hipLaunchKernelGGL(k_hsml<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
 }
 
+// The general path with a candidate list. Head: reference grid, predict +
+// EOS + records + reference cells + displacement vote (mode bits as
+// k_point_gen). Tail: the rebuild (cell sort and candidate list) when the
+// gate is open, this step's exact list from the candidates, the pair and
+// force passes. count_only: size the candidate list and stop.
+template <int DIM>
+static void gen_head(sphgpu_ctx* c, int mode) {
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL(k_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, mode);
+}
+
+template <int DIM>
+static void gen_tail(sphgpu_ctx* c, int count_only) {
+    Params& P = c->P;
+    const int nb = (int)((P.n + kBlock - 1) / kBlock);
+    const Gate G = fast_gate(c);
+    launch_sort<DIM>(c, G, 0);
+    hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only,
+                       RunMap{});
+    if (count_only) return;
+    hipLaunchKernelGGL(k_filter<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    launch_physics<DIM>(c);
+}
+
+template <int DIM>
+static void launch_derivs(sphgpu_ctx* c, int mode) {
+    if (fluid_fast_path(c) && c->use_verlet) {
+        launch_fast<DIM>(c, mode);
+        return;
+    }
+    if (c->use_verlet) {
+        gen_head<DIM>(c, mode);
+        gen_tail<DIM>(c, 0);
+        return;
+    }
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    launch_sort<DIM>(c);
+    launch_point_build<DIM>(c, mode);
+    launch_physics<DIM>(c);
+}
+
+// One step of the exact-list paths: search, physics, correct.
 template <int DIM>
 static void launch_step(sphgpu_ctx* c) {
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
@@ -2373,11 +2411,57 @@ static void launch_step(sphgpu_ctx* c) {
         hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first, RunMap{}, 0);
 }
 
-// nsteps steps of the general path: the first step of a run eager, then
-// whole groups of kGraphSteps replaying a captured graph while more than one
-// step is left, and the rest eager, the last one with its force pass timed.
+// A step of the general path with a candidate list that is followed by
+// another: its tail, the correct fused with the next step's predict, and the
+// next step's reference grid and cells.
+template <int DIM>
+static void gen_body_step(sphgpu_ctx* c) {
+    const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+    const int first = c->step == 1;
+    gen_tail<DIM>(c, 0);
+    hipLaunchKernelGGL(k_correct_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(std::min(nb, kStrideBlocks)), dim3(kBlock), 0, c->stream,
+                       c->P, c->D, RunMap{});
+}
+
+// nsteps steps of the general path. With a candidate list: the head of the
+// first step, nsteps - 1 body steps (whole groups of kGraphSteps replay a
+// captured graph once the first step is past) and a closing step whose force
+// pass is timed, as on the fast path. Exact lists: whole steps, groups of
+// kGraphSteps replayed while more than one step is left.
 template <int DIM>
 static int advance_general(sphgpu_ctx* c, int nsteps) {
+    if (nsteps <= 0) return 0;
+    if (c->use_verlet) {
+        if (c->use_graph && !c->graph) {
+            int rc = capture_steps(c, [c] { gen_body_step<DIM>(c); });
+            if (rc) return rc;
+        }
+        gen_head<DIM>(c, c->step == 1 ? 1 : 0);
+        int body = nsteps - 1;
+        if (c->use_graph && c->step > 1 && body >= kGraphSteps) {
+            const int reps = body / kGraphSteps;
+            for (int r = 0; r < reps; ++r) {
+                HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+                c->step += kGraphSteps;
+            }
+            body -= reps * kGraphSteps;
+        }
+        for (int k = 0; k < body; ++k) {
+            gen_body_step<DIM>(c);
+            c->step++;
+        }
+        const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
+        const int first = c->step == 1;
+        c->time_force = true;
+        gen_tail<DIM>(c, 0);
+        c->time_force = false;
+        hipLaunchKernelGGL((k_correct<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first,
+                           RunMap{}, 0);
+        c->step++;
+        return 0;
+    }
     if (c->use_graph && !c->graph) {
         int rc = capture_steps(c, [c] { launch_step<DIM>(c); });
         if (rc) return rc;
@@ -2565,7 +2649,8 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     } else if (c->use_verlet) {
         HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
         force_rebuild(c);
-        launch_gen_search<DIM>(c, 2 | 4, 1);
+        gen_head<DIM>(c, 2 | 4);
+        gen_tail<DIM>(c, 1);
         HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         e = check_err(c);
