@@ -49,30 +49,19 @@ __global__ __launch_bounds__(kBlock) void k_point_gen(Params P, Dev D, int mode)
     vote_rebuild(P, moved);
 }
 
-// Exact neighbour list of this step from the candidate list: the reference's
-// distance test and window on every candidate, kept in candidate order (the
-// order of the cell sweep at the build), in the wave-blocked layout the pair
-// kernels read. The distance is tested in FP32 from the cell-offset records
-// first; only a candidate within the FP32 error bound of the cutoff (or off
-// the grid) loads its FP64 position for the reference's test. A particle whose
-// candidates outgrew its list walks its cell window of the last sort instead.
-// Closes the rebuild gate of the step: every gated kernel ran before this
-// launch.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        P.vflag[0] = 0ull;
-        P.vflag[1] = 0ull;
-    }
-    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= P.n) return;
-    const int4 mt = D.meta[s];
-    if (mt.w >= P.n_real) return;
-    if (mt.y == -1) {
-        D.ncnt[s] = 0;
-        return;
-    }
-    const double4 qi = D.posh[s];
+// Exact neighbour list of target s for this step from its candidate list:
+// the reference's distance test and window on every candidate, kept in
+// candidate order (the order of the cell sweep at the build), in the
+// wave-blocked layout the pair kernels read. The distance is tested in FP32
+// from the cell-offset records first; only a candidate within the FP32 error
+// bound of the cutoff (or off the grid) loads its FP64 position for the
+// reference's test. A particle whose candidates outgrew its list walks its
+// cell window of the last sort instead. visit(j, pf_j) is called for every
+// accepted neighbour in list order (pf_j its cell-offset record). Returns the
+// neighbour count (the list holds at most nbr_cap of them).
+template <int DIM, typename V>
+__device__ __forceinline__ int filter_target(const Params& P, const Dev& D, int s, const double4& qi,
+                                             V&& visit) {
     const double cut = qi.w * (double)P.scale_k;
     const double cut2 = cut * cut;
     const double lo2 = cut2 * (1.0 - 1e-12), hi2 = cut2 * (1.0 + 1e-12);
@@ -122,6 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
         }
         if (cnt < cap) D.nbr[vlist_at(s, cnt, cap)] = j;
         ++cnt;
+        visit(j, pf);
     };
     const int nc = D.vcnt[s];
     if (nc > P.verlet_cap) {
@@ -156,9 +146,41 @@ __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
             pj = pn;
         }
     }
+    return cnt;
+}
+
+// The count of a target's exact list and its checks: capacity (the step's
+// sums would be incomplete) and the largest count.
+__device__ __forceinline__ void filter_done(const Params& P, const Dev& D, int s, int cnt) {
     D.ncnt[s] = cnt;
-    if (cnt > cap) atomicOr(P.err, kErrPairCap);
+    if (cnt > P.nbr_cap) atomicOr(P.err, kErrPairCap);
     if (cnt > __atomic_load_n(P.maxcnt, __ATOMIC_RELAXED)) atomicMax(P.maxcnt, cnt);
+}
+
+// Clears the rebuild gate of the step: every gated kernel ran before the
+// launch that calls it (the filter, alone or in the strain pass).
+__device__ __forceinline__ void close_gate(const Params& P) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.vflag[0] = 0ull;
+        P.vflag[1] = 0ull;
+    }
+}
+
+// The exact lists of this step for every real target (filter_target), when
+// no pair pass filters the candidates itself.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
+    close_gate(P);
+    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const int4 mt = D.meta[s];
+    if (mt.w >= P.n_real) return;
+    if (mt.y == -1) {
+        D.ncnt[s] = 0;
+        return;
+    }
+    const int cnt = filter_target<DIM>(P, D, s, D.posh[s], [](int, const float4&) {});
+    filter_done(P, D, s, cnt);
 }
 
 // The step boundary of the general path: the correct of step n
@@ -189,4 +211,38 @@ __global__ __launch_bounds__(kBlock) void k_correct_point_gen(Params P, Dev D, i
     }
     vote_rebuild(P, moved);
     bbox_block_reduce(mn, mx, P.partial, blockIdx.x);
+}
+
+// The first pair pass of a general step with the filter in it: each real
+// target's exact list from its candidates (filter_target) and its
+// strain-rate sums over the neighbours as they are accepted, so each
+// neighbour's cell-offset record is loaded once. The list order, and so
+// every sum, is that of k_filter followed by k_strain. One lane per target;
+// no kernel gradient correction (its pass comes first when it is on).
+template <int DIM, typename R>
+__global__ __launch_bounds__(kBlock) void k_strain_filter(Params P, Dev D) {
+    close_gate(P);
+    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const int4 mt = D.meta[s];
+    const Material* m = find_mat(P, mt.x);
+    const double eta = (P.viscosity && m) ? m->eta : 0.0;   // viscosity.f90:14-33
+    D.eta[s] = eta;
+    R T[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) T[q] = (R)0.0;
+    if (mt.w < P.n_real) {
+        if (mt.y == -1) {
+            D.ncnt[s] = 0;
+        } else {
+            const double4 qi = D.posh[s];
+            const double4 vi = D.vm[s];
+            const R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
+            const float4 pfi = D.posf[s];
+            const int cnt = filter_target<DIM>(P, D, s, qi, [&](int j, const float4& pfj) {
+                strain_pair<DIM, R>(P, D, j, &pfj, qi, vi, pfi, f, nullptr, T);
+            });
+            filter_done(P, D, s, cnt);
+        }
+    }
+    strain_finish<DIM, R>(P, D, s, mt, eta, T);
 }

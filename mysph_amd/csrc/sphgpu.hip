@@ -960,65 +960,44 @@ __global__ __launch_bounds__(kBlock) void k_apply_sumdens(Params P, Dev D) {
     if (P.pair32) write_rec32(D, s);
 }
 
-// Strain-rate (and solid rotation-rate) sums, internalForce.f90:48-100, and the
-// fluid viscous heating tdsdt (:108-112). T(d,e) = sum_j V_j (v_j - v_i)_d dW_e.
-template <int DIM, typename R, int LPT, bool KGC>
-__global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
-    int sub;
-    const int s = target_slot<LPT>(P, nullptr, 0, sub);
-    if (s >= P.n) return;
-    int4 mt = D.meta[s];
-    const Material* m = find_mat(P, mt.x);
-    double eta = (P.viscosity && m) ? m->eta : 0.0;   // viscosity.f90:14-33
-    D.eta[s] = eta;
-    R T[DIM * DIM];
-    for (int q = 0; q < DIM * DIM; ++q) T[q] = (R)0.0;
-    bool real = mt.w < P.n_real && mt.y != -1;
-    if (real) {
-        // mixed precision (R = float) gathers the FP32 pair records
-        constexpr bool REC32 = std::is_same<R, float>::value;
-        double4 qi = D.posh[s];
-        double4 vi = D.vm[s];
-        R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
-        // KGC: the run corrects kernel gradients (compiled out otherwise)
-        const double* A = KGC ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
-        float4 pfi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (REC32) pfi = D.posf[s];
-        const float csf = (float)P.cs;
-        int n = min(D.ncnt[s], P.nbr_cap);
-        // the next index is in flight while a pair is summed
-        int jn = sub < n ? D.nbr[vlist_at(s, sub, P.nbr_cap)] : 0;
-        for (int k = sub; k < n; k += LPT) {
-            const int j = jn;
-            if (k + LPT < n) jn = D.nbr[vlist_at(s, k + LPT, P.nbr_cap)];
-            PairGeo<DIM, R> g;
-            R V, dvn[3];
-            if constexpr (REC32) {
-                const float4 pfj = D.posf[j];
-                const float4 vj = D.prf[j].vm;
-                rec_dx<DIM>(pfi, pfj, csf, g.dx);
-                pair_geo_dx<DIM, R>(P, (R)qi.w, f, A, g);
-                V = vj.w;
-                dvn[0] = vj.x - (R)vi.x;
-                dvn[1] = vj.y - (R)vi.y;
-                dvn[2] = vj.z - (R)vi.z;
-            } else {
-                double4 qj = D.posh[j];
-                double4 vj = D.vm[j];
-                pair_geo<DIM, R>(P, qi, qj, f, A, g);
-                V = (R)D.th[j].w;   // m / rho of this step's EOS
-                dvn[0] = (R)(vj.x - vi.x);
-                dvn[1] = (R)(vj.y - vi.y);
-                dvn[2] = (R)(vj.z - vi.z);
-            }
-            for (int d = 0; d < DIM; ++d)
-                for (int e = 0; e < DIM; ++e) T[d * DIM + e] += V * dvn[d] * g.gw[e];
-        }
+// Strain-rate sum of one pair (target s with FP64 state qi, vi; its FP32
+// record pfi in mixed precision, R = float, which gathers the neighbour's
+// FP32 records; pfj the neighbour's, when the caller has it).
+template <int DIM, typename R>
+__device__ __forceinline__ void strain_pair(const Params& P, const Dev& D, int j, const float4* pfj_in,
+                                            const double4& qi, const double4& vi, const float4& pfi, R f,
+                                            const double* A, R T[DIM * DIM]) {
+    constexpr bool REC32 = std::is_same<R, float>::value;
+    PairGeo<DIM, R> g;
+    R V, dvn[3];
+    if constexpr (REC32) {
+        const float4 pfj = pfj_in ? *pfj_in : D.posf[j];
+        const float4 vj = D.prf[j].vm;
+        rec_dx<DIM>(pfi, pfj, (float)P.cs, g.dx);
+        pair_geo_dx<DIM, R>(P, (R)qi.w, f, A, g);
+        V = vj.w;
+        dvn[0] = vj.x - (R)vi.x;
+        dvn[1] = vj.y - (R)vi.y;
+        dvn[2] = vj.z - (R)vi.z;
+    } else {
+        double4 qj = D.posh[j];
+        double4 vj = D.vm[j];
+        pair_geo<DIM, R>(P, qi, qj, f, A, g);
+        V = (R)D.th[j].w;   // m / rho of this step's EOS
+        dvn[0] = (R)(vj.x - vi.x);
+        dvn[1] = (R)(vj.y - vi.y);
+        dvn[2] = (R)(vj.z - vi.z);
     }
-    if (LPT > 1) {
-        for (int q = 0; q < DIM * DIM; ++q) T[q] = group_sum<LPT>(T[q]);
-        if (sub) return;
-    }
+    for (int d = 0; d < DIM; ++d)
+        for (int e = 0; e < DIM; ++e) T[d * DIM + e] += V * dvn[d] * g.gw[e];
+}
+
+// Strain-rate results of target s from its sums T: the fluid's viscous
+// stress term and heating (internalForce.f90:48-112), or the solid's strain
+// and rotation rates; eta of the target's material (viscosity.f90:14-33).
+template <int DIM, typename R>
+__device__ __forceinline__ void strain_finish(const Params& P, const Dev& D, int s, int4 mt, double eta,
+                                              const R T[DIM * DIM]) {
     int ai = mt.x < 0 ? -mt.x : mt.x;
     double rho = D.th[s].x;
     int64_t o = (int64_t)s * DIM * DIM;
@@ -1044,6 +1023,45 @@ __global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
                 if (P.pair32) D.Epf[o + d * DIM + e] = (float)D.Ep[o + d * DIM + e];
             }
     }
+}
+
+// Strain-rate (and solid rotation-rate) sums over the exact list,
+// internalForce.f90:48-100, and the fluid viscous heating tdsdt (:108-112).
+// T(d,e) = sum_j V_j (v_j - v_i)_d dW_e.
+template <int DIM, typename R, int LPT, bool KGC>
+__global__ __launch_bounds__(kBlock) void k_strain(Params P, Dev D) {
+    int sub;
+    const int s = target_slot<LPT>(P, nullptr, 0, sub);
+    if (s >= P.n) return;
+    int4 mt = D.meta[s];
+    const Material* m = find_mat(P, mt.x);
+    double eta = (P.viscosity && m) ? m->eta : 0.0;   // viscosity.f90:14-33
+    D.eta[s] = eta;
+    R T[DIM * DIM];
+    for (int q = 0; q < DIM * DIM; ++q) T[q] = (R)0.0;
+    bool real = mt.w < P.n_real && mt.y != -1;
+    if (real) {
+        const double4 qi = D.posh[s];
+        const double4 vi = D.vm[s];
+        const R f = kernel_factor<DIM, R>(P.skf, (R)qi.w);
+        // KGC: the run corrects kernel gradients (compiled out otherwise)
+        const double* A = KGC ? D.kgcA + (int64_t)s * DIM * DIM : nullptr;
+        float4 pfi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (std::is_same<R, float>::value) pfi = D.posf[s];
+        int n = min(D.ncnt[s], P.nbr_cap);
+        // the next index is in flight while a pair is summed
+        int jn = sub < n ? D.nbr[vlist_at(s, sub, P.nbr_cap)] : 0;
+        for (int k = sub; k < n; k += LPT) {
+            const int j = jn;
+            if (k + LPT < n) jn = D.nbr[vlist_at(s, k + LPT, P.nbr_cap)];
+            strain_pair<DIM, R>(P, D, j, nullptr, qi, vi, pfi, f, A, T);
+        }
+    }
+    if (LPT > 1) {
+        for (int q = 0; q < DIM * DIM; ++q) T[q] = group_sum<LPT>(T[q]);
+        if (sub) return;
+    }
+    strain_finish<DIM, R>(P, D, s, mt, eta, T);
 }
 
 // Solid point pass, internalForce.f90:137-180 and artificialTerm.f90:42-69.
@@ -1771,6 +1789,7 @@ struct sphgpu_ctx {
     bool use_verlet = true;       // candidate lists (SPHGPU_NO_VERLET=1: exact lists by a cell sweep every step)
     bool fused_done = false;      // a step has run since the upload: the candidate list is valid
     bool use_graph = true;        // replay captured step sequences (SPHGPU_NO_GRAPH=1: eager)
+    bool separate_filter = false; // testing switch: keep the candidate filter a pass of its own
     hipGraphExec_t graph = nullptr;   // kGraphSteps fast-path steps, valid until the next upload
     std::string err;
     int* h_err = nullptr;   // pinned
@@ -1910,6 +1929,8 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
     c->use_verlet = !(nv && nv[0] == '1');
     const char* ng = getenv("SPHGPU_NO_GRAPH");
     c->use_graph = !(ng && ng[0] == '1');
+    const char* sf = getenv("SPHGPU_SEPARATE_FILTER");
+    c->separate_filter = sf && sf[0] == '1';
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         c->stream = nullptr;
@@ -2266,6 +2287,17 @@ static void launch_point_build(sphgpu_ctx* c, int mode) {
         hipLaunchKernelGGL((k_point_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, mode);
 }
 
+// In mixed precision the candidate filter runs inside the strain pass when
+// that is the step's first pair pass (no kernel gradient correction or
+// summation density before it) and takes one lane per target; otherwise it is
+// a pass of its own. (FP64 runs it alone: the fused FP64 pass holds too many
+// registers to gain.) SPHGPU_SEPARATE_FILTER=1 keeps it alone (testing).
+static bool strain_filters(const sphgpu_ctx* c) {
+    const Params& P = c->P;
+    return c->use_verlet && !c->separate_filter && P.pair32 && !fluid_fast_path(c) && !P.kgc &&
+           !P.summation_density && (P.viscosity || P.has_solid) && P.n_real > kSmallRun;
+}
+
 // The general path's pair passes before the force pass: kernel gradient
 // correction, summation density, strain rates, with LPT lanes per target.
 template <int DIM, typename R, int LPT>
@@ -2280,6 +2312,8 @@ static void launch_pair_passes(sphgpu_ctx* c) {
     }
     if ((P.viscosity || P.has_solid) && P.kgc)
         hipLaunchKernelGGL((k_strain<DIM, R, LPT, true>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+    else if (LPT == 1 && strain_filters(c))
+        hipLaunchKernelGGL((k_strain_filter<DIM, R>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     else if (P.viscosity || P.has_solid)
         hipLaunchKernelGGL((k_strain<DIM, R, LPT, false>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
 }
@@ -2378,7 +2412,7 @@ static void gen_tail(sphgpu_ctx* c, int count_only) {
     hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only,
                        RunMap{});
     if (count_only) return;
-    hipLaunchKernelGGL(k_filter<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (!strain_filters(c)) hipLaunchKernelGGL(k_filter<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     launch_physics<DIM>(c);
 }
 

@@ -15,7 +15,10 @@ steps replay a captured graph.
     with and without an evolving h;
   - particles whose candidates outgrow their list (SPHGPU_VERLET_CAP) walk
     their cell window and stay exact;
-  - split calls and graph replays equal one eager call bit for bit.
+  - split calls and graph replays equal one eager call bit for bit;
+  - a run past the small-run size (one lane per fluid target, solid
+    targets on a second stream, the mixed filter inside the strain pass)
+    against the oracle, and its fused filter against the separate one.
 """
 from __future__ import annotations
 
@@ -166,3 +169,53 @@ def test_general_graph_replays_equal_eager_launches(which, precision, monkeypatc
                     sim.advance(34)          # 3 graph replays + 4 eager steps
                 for k in STATE_FIELDS + ("stress",) * (which == "gate"):
                     assert np.array_equal(g.get(k), e.get(k)), (rep, k)
+
+
+def _mid_gate():
+    """The elastic gate past the small-run size (291,750 real particles): one
+    lane per fluid target, the solid targets on the side stream with several
+    lanes each, and in mixed precision the filter inside the strain pass."""
+    case = cases.dam_break_gate_3d(dx_f=0.0025, width=0.3)
+    assert case["n_real"] > 262_144
+    return case
+
+
+def test_large_general_run_vs_oracle():
+    case = _mid_gate()
+    nt = case["n_real"]
+    print("large general run: step-1 derivatives and 10 steps against the oracle", flush=True)
+    orc = oracle.OracleSim(case)
+    with Simulation(case) as sim:
+        sim.derivatives()
+        orc.derivatives()
+        for k in ("dvdt", "drhodt", "dedt", "dsdt"):
+            assert_close("d1 " + k, sim.get(k)[:nt], orc[k][:nt], 1e-10)
+    orc = oracle.OracleSim(case)
+    orc.run(10)
+    pert = dict(case)
+    pert["v"] = case["v"] + 1e-15 * np.random.default_rng(3).standard_normal(case["v"].shape)
+    o2 = oracle.OracleSim(pert)
+    o2.run(10)
+    with Simulation(case) as sim:
+        sim.advance(10)
+        for k in STATE_FIELDS + ("stress",):
+            tol = max(1e-9, 10 * rel_err(o2[k][:nt], orc[k][:nt]))
+            assert_close(k, sim.get(k)[:nt], orc[k][:nt], tol)
+
+
+def test_large_general_run_fused_filter_equals_separate(monkeypatch):
+    """Mixed precision: the filter inside the strain pass writes the same
+    exact lists in the same order as the filter pass of its own, so every
+    field is bit-identical over 40 steps (rebuilds included)."""
+    case = _mid_gate()
+    cfg = case["config"].copy()
+    cfg.precision = C.MIXED
+    print("large general run, mixed: fused vs separate filter", flush=True)
+    with Simulation(case, cfg) as a:
+        a.advance(40)
+        ra = {k: a.get(k) for k in STATE_FIELDS + ("stress",)}
+    monkeypatch.setenv("SPHGPU_SEPARATE_FILTER", "1")
+    with Simulation(case, cfg) as b:
+        b.advance(40)
+        for k in STATE_FIELDS + ("stress",):
+            assert np.array_equal(b.get(k), ra[k]), k
