@@ -41,13 +41,15 @@ typedef enum sphgpu_status {
     SPHGPU_E_OUT_OF_RANGE = -1,   /* particle outside the grid (asymmetricSearch.f90:38-39) */
     SPHGPU_E_NONFINITE = -2,      /* NaN/Inf position, density or velocity */
     SPHGPU_E_PAIR_CAPACITY = -3,  /* pair export buffer too small; or, from a step on
-                                   * the general path or in a batch, a particle outgrew
-                                   * the neighbour lists planned at upload (1.5x the
-                                   * largest count + 8; the reference grows its list):
-                                   * that step's sums are incomplete, upload the last
-                                   * good state again. A single run's fluid fast path
-                                   * walks the cell window of such a particle instead
-                                   * and stays exact. */
+                                   * the general path, a particle outgrew the exact
+                                   * neighbour list planned at upload (1.5x the largest
+                                   * count + 8; the reference grows its list): that
+                                   * step's sums are incomplete, upload the last good
+                                   * state again. A particle that outgrows its candidate
+                                   * list (fast path, single or batched, and general
+                                   * path) walks its cell window instead and stays
+                                   * exact; a batched run rebuilds every step while it
+                                   * has one. */
     SPHGPU_E_BAD_CONFIG = -4,     /* unsupported switch combination */
     SPHGPU_E_HIP = -5,            /* HIP runtime error */
     SPHGPU_E_ARG = -6,            /* bad argument (NULL, size) */

@@ -11,6 +11,15 @@
 // the reference's arithmetic (asymmetricSearch.f90:227-237) and window
 // (:209-217).
 
+// A batched run keeps its cell tables only while it rebuilds: a real particle
+// whose candidates outgrew its list (it walks its cell window in the fused
+// pass) votes for the next step's rebuild, so the run rebuilds every step
+// until its lists fit again.
+template <bool BATCH>
+__device__ __forceinline__ bool overflow_vote(const Params& P, const Dev& D, int s, int4 mt) {
+    return BATCH && mt.w < P.n_real && mt.y != -1 && D.vcnt[s] > P.verlet_cap;
+}
+
 // Predict + EOS + pair inputs for every particle, the reference cell (for the
 // window check), and the largest displacement since the last list build.
 template <int DIM, bool MIXED, bool BATCH>
@@ -44,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_point_fast(Params P0, Dev D, int mod
                                                                                       : kErrNonFinite);
             D.refcell[s] = pack_ref(rc[0], rc[1], rc[2]);
         }
-        moved = moved_half_skin<MIXED>(P, D, s, q);
+        moved = moved_half_skin<MIXED>(P, D, s, q) || overflow_vote<BATCH>(P, D, s, mt);
     }
     // one atomic per wave, and only when some lane crossed the threshold
     vote_rebuild(P, moved);
@@ -130,9 +139,6 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
             y = ny; z = nz; kb = nkb; ke = nke;
         }
         D.vcnt[s] = cnt;
-        // one run's force pass walks the cell window of a particle whose list
-        // overflowed; a batch's cannot (see k_force_fused)
-        if (BATCH && !count_only && cnt > P.verlet_cap) atomicOr(P.err, kErrPairCap);
     }
     // one atomic per wave for the largest count (same-address atomics from
     // every lane serialise in L2)
@@ -232,14 +238,14 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
     const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, psi2 = (R)(P.av_psi * P.av_psi);
     R ax = 0, ay = 0, az = 0, de_in = 0, de_av = 0, sv = 0, sx = 0;
     int acc_n = 0;
-    // a particle with more candidates than its list holds (one run) walks its
-    // cell window of the last sort instead: that window holds every
-    // candidate the list would (the sort is at or after the list's build),
-    // and the exact test keeps only the reference's pairs. A batch's shared
-    // cell tables do not hold a run that is not rebuilding, so there the
-    // list build reports the overflow instead.
+    // a particle with more candidates than its list holds walks its cell
+    // window of the last sort instead: that window holds every candidate the
+    // list would (the sort is at or after the list's build), and the exact
+    // test keeps only the reference's pairs. A batch's shared cell tables
+    // hold a run only in the steps it rebuilds, so a batched run with such a
+    // particle votes for a rebuild every step (overflow_vote).
     const int nc = (mt.y == -1) ? 0 : D.vcnt[s];
-    const bool sweep = !BATCH && nc > P.verlet_cap;
+    const bool sweep = nc > P.verlet_cap;
     const int n = sweep ? 0 : min(nc, P.verlet_cap);
     const int* __restrict__ nl = D.vnbr + vlist_at(s, 0, P.verlet_cap);
     // the mixed pair sums in fewer operations (the same quantities, FP32
@@ -512,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
                 atomicOr(P.err, kErrNonFinite);
             if (MIXED) eos_mixed_vals(P, D, s, mt.x, dr_pred, v, q);
             else eos_th(P, D, s, mt);
-            moved = moved_half_skin<MIXED>(P, D, s, q);
+            moved = moved_half_skin<MIXED>(P, D, s, q) || overflow_vote<BATCH>(P, D, s, mt);
         }
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;
     }
