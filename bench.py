@@ -341,6 +341,9 @@ def serving_leg(args):
         "arena_slots": args.serving_slots,
         "chunk_steps": args.serving_chunk,
         "tiling_per_size_class": {"bounds": list(tuner.bounds), "win_skin": tuner.table,
+                                  "differs_by_class": len(set(map(tuple, tuner.table))) > 1,
+                                  "margin_over_second": {str(k): (v[1][2] / v[0][2] - 1.0) if len(v) > 1 else None
+                                                         for k, v in report.items()},
                                   "ns_per_particle_step": {str(k): v for k, v in report.items()}},
         "headline_rate_factor": head.get("rate_factor", "burst"),
         "p50_s": head["p50_s"],
