@@ -12,7 +12,9 @@ Prints ONE JSON line. ``value`` is the mixed-precision particle-steps/s
 difference between them, the force kernel's roofline, the per-call rate of
 the drop-in single_step with host buffers (PCIe and host work included), the
 batched serving leg and a CPU baseline (the oracle restatement on the host
-cores, on a bounded sample) ride along.
+cores, on a bounded sample, calibrated against the compiled reference) ride
+along; --general (or --only general) adds the general-path workloads
+(elastic beam, elastic gate).
 """
 from __future__ import annotations
 
@@ -282,6 +284,32 @@ def cpu_baseline(case, seconds_target=20.0):
     return out
 
 
+def general_leg():
+    """The general path's workloads (secondary lines, not the headline): the
+    2D elastic beam (16,120 particles, artificial stress, clamped columns) and
+    the 3D dam break with an elastic gate (2,046,000 fluid + 45,900 solid +
+    467,742 fixed particles, physical viscosity, fluid-solid coupling, h per
+    phase), each in FP64 and in mixed precision, timed after a warm-up."""
+    from mysph_amd import cases
+    from mysph_amd import config as C
+    out = {}
+    for name, make, warm, steps in (("elastic_beam_2d", cases.elastic_beam, 20, 2000),
+                                    ("elastic_gate_3d", cases.dam_break_gate_3d, 10, 100)):
+        case = make()
+        nt = case["n_real"]
+        rec = {"real_particles": int(nt), "fixed_particles": int(case["n_dummy"]), "steps": steps,
+               "warmup": warm}
+        for pname, prec in (("fp64", C.FP64), ("mixed", C.MIXED)):
+            r = run_gpu(case, prec, warm, steps)
+            rec[pname] = {"ms_per_step": 1e3 * r["wall_s"] / steps, "particle_steps_per_s": nt * steps / r["wall_s"],
+                          "force_pass_ms": r["force_ms"]}
+        rec["mixed_over_fp64"] = rec["fp64"]["ms_per_step"] / rec["mixed"]["ms_per_step"]
+        out[name] = rec
+        log(f"{name}: fp64 {rec['fp64']['ms_per_step']:.3f} ms/step, mixed {rec['mixed']['ms_per_step']:.3f} "
+            f"ms/step ({rec['mixed_over_fp64']:.2f}x)")
+    return out
+
+
 def serving_leg(args):
     """Continuous batching of the dam-break parameter sweep: tune the
     per-size tiling, find the saturation rate with all requests arriving at
@@ -363,8 +391,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.0075)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["both", "mixed", "fp64", "serving"], default="both",
-                    help="serving: only the batched serving leg")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64", "serving", "general"], default="both",
+                    help="serving: only the batched serving leg; general: only the general-path workloads")
+    ap.add_argument("--general", action="store_true",
+                    help="also time the general-path workloads (elastic beam, elastic gate)")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed run of both precisions to report the mixed density error")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched serving leg")
@@ -397,6 +427,9 @@ def main():
 
     if args.only == "serving":
         print(json.dumps({"serving": serving_leg(args)}), flush=True)
+        return
+    if args.only == "general":
+        print(json.dumps({"general": general_leg()}), flush=True)
         return
     from mysph_amd import cases
     from mysph_amd import config as C
@@ -486,6 +519,8 @@ def main():
     if not args.no_pcie:
         out["pcie_inclusive"] = pcie_inclusive(case)
         log(f"drop-in single_step with host buffers: {out['pcie_inclusive']['ms_per_step']:.2f} ms/step")
+    if args.general:
+        out["general"] = general_leg()
     if not args.no_serving:
         out["serving"] = serving_leg(args)
     if not args.no_cpu:
