@@ -284,7 +284,7 @@ def cpu_baseline(case, seconds_target=20.0):
     return out
 
 
-def general_leg():
+def general_leg(beam_steps=2000, gate_steps=100):
     """The general path's workloads (secondary lines, not the headline): the
     2D elastic beam (16,120 particles, artificial stress, clamped columns) and
     the 3D dam break with an elastic gate (2,046,000 fluid + 45,900 solid +
@@ -293,8 +293,8 @@ def general_leg():
     from mysph_amd import cases
     from mysph_amd import config as C
     out = {}
-    for name, make, warm, steps in (("elastic_beam_2d", cases.elastic_beam, 20, 2000),
-                                    ("elastic_gate_3d", cases.dam_break_gate_3d, 10, 100)):
+    for name, make, warm, steps in (("elastic_beam_2d", cases.elastic_beam, 20, beam_steps),
+                                    ("elastic_gate_3d", cases.dam_break_gate_3d, 10, gate_steps)):
         case = make()
         nt = case["n_real"]
         rec = {"real_particles": int(nt), "fixed_particles": int(case["n_dummy"]), "steps": steps,
@@ -395,6 +395,8 @@ def main():
                     help="serving: only the batched serving leg; general: only the general-path workloads")
     ap.add_argument("--general", action="store_true",
                     help="also time the general-path workloads (elastic beam, elastic gate)")
+    ap.add_argument("--general-steps", type=int, nargs=2, default=[2000, 100], metavar=("BEAM", "GATE"),
+                    help="timed steps of the elastic beam and of the elastic gate")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed run of both precisions to report the mixed density error")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched serving leg")
@@ -429,7 +431,7 @@ def main():
         print(json.dumps({"serving": serving_leg(args)}), flush=True)
         return
     if args.only == "general":
-        print(json.dumps({"general": general_leg()}), flush=True)
+        print(json.dumps({"general": general_leg(*args.general_steps)}), flush=True)
         return
     from mysph_amd import cases
     from mysph_amd import config as C
@@ -520,7 +522,7 @@ def main():
         out["pcie_inclusive"] = pcie_inclusive(case)
         log(f"drop-in single_step with host buffers: {out['pcie_inclusive']['ms_per_step']:.2f} ms/step")
     if args.general:
-        out["general"] = general_leg()
+        out["general"] = general_leg(*args.general_steps)
     if not args.no_serving:
         out["serving"] = serving_leg(args)
     if not args.no_cpu:

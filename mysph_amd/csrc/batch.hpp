@@ -455,7 +455,7 @@ static void batch_chunk(sphgpu_batch* b, int m) {
     const int sb = (mc + kScanTile - 1) / kScanTile;
     const Gate G{b->d_vany};
     hipLaunchKernelGGL(k_bbox<true>, dim3(nbt), dim3(kBlock), 0, st, P0, D, M);
-    hipLaunchKernelGGL((k_grid<DIM, true>), dim3(nr), dim3(kBlock), 0, st, P0, D, M);
+    hipLaunchKernelGGL((k_grid<DIM, true>), dim3(nr), dim3(kGridThreads), 0, st, P0, D, M);
     hipLaunchKernelGGL((k_point_fast<DIM, MIXED, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, 0, M);
     for (int j = 0; j < m; ++j) {
         hipLaunchKernelGGL((k_keys<DIM, true>), dim3(nss), dim3(kBlock), 0, st, P0, D, G, 0, M);
@@ -470,7 +470,7 @@ static void batch_chunk(sphgpu_batch* b, int m) {
         hipLaunchKernelGGL((k_force_fused<DIM, MIXED, false, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, M);
         if (j + 1 < m) {
             hipLaunchKernelGGL((k_correct_predict<DIM, MIXED, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, 0, M, j);
-            hipLaunchKernelGGL((k_grid<DIM, true>), dim3(nr), dim3(kBlock), 0, st, P0, D, M);
+            hipLaunchKernelGGL((k_grid<DIM, true>), dim3(nr), dim3(kGridThreads), 0, st, P0, D, M);
             hipLaunchKernelGGL((k_refcell<DIM, true>), dim3(nss), dim3(kBlock), 0, st, P0, D, M);
         } else {
             hipLaunchKernelGGL((k_correct<DIM, MIXED, true>), dim3(nbt), dim3(kBlock), 0, st, P0, D, 0, M, j);

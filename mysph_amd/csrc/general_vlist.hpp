@@ -59,9 +59,15 @@ __global__ __launch_bounds__(kBlock) void k_point_gen(Params P, Dev D, int mode)
 // cell window of the last sort instead. visit(j, pf_j) is called for every
 // accepted neighbour in list order (pf_j its cell-offset record). Returns the
 // neighbour count (the list holds at most nbr_cap of them).
-template <int DIM, typename V>
+//
+// LPT > 1 (small runs, no visitor): the LPT lanes of a group (sub = 0..LPT-1)
+// test LPT consecutive candidates at once; a ballot over the group gives each
+// accepted one its place, so the list is the same as one lane's. A particle
+// whose candidates overflowed walks its cell window on lane 0 of its group.
+// The returned count is lane 0's.
+template <int DIM, int LPT = 1, typename V>
 __device__ __forceinline__ int filter_target(const Params& P, const Dev& D, int s, const double4& qi,
-                                             V&& visit) {
+                                             V&& visit, int sub = 0) {
     const double cut = qi.w * (double)P.scale_k;
     const double cut2 = cut * cut;
     const double lo2 = cut2 * (1.0 - 1e-12), hi2 = cut2 * (1.0 + 1e-12);
@@ -93,7 +99,8 @@ __device__ __forceinline__ int filter_target(const Params& P, const Dev& D, int 
     }
     const int cap = P.nbr_cap;
     int cnt = 0;
-    auto take = [&](int j, const float4& pf) {
+    // the reference's rule for candidate j (pf: its cell-offset record)
+    auto accept = [&](int j, const float4& pf) -> bool {
         const uint32_t pk = __float_as_uint(pf.w);
         int cxj, cyj, czj;
         unpack_cell<DIM>(pk, cxj, cyj, czj);
@@ -101,19 +108,45 @@ __device__ __forceinline__ int filter_target(const Params& P, const Dev& D, int 
         const float dy = (pfi.y - pf.y) + (float)(cyi - cyj) * csf;
         const float dz = DIM == 3 ? (pfi.z - pf.z) + (float)(czi - czj) * csf : 0.0f;
         const float r2 = (int)(pk | pk_i) < 0 ? lof : dx * dx + dy * dy + dz * dz;
-        if (!(r2 < hif)) return;
-        if (!(r2 < lof) && exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) return;
+        if (!(r2 < hif)) return false;
+        if (!(r2 < lof) && exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) < 0.0) return false;
         if (need_win) {
             const uint32_t rcj = D.refcell[j];
             bool inwin = true;
             for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
-            if (!inwin) return;
+            if (!inwin) return false;
         }
+        return true;
+    };
+    auto take = [&](int j, const float4& pf) {
+        if (!accept(j, pf)) return;
         if (cnt < cap) D.nbr[vlist_at(s, cnt, cap)] = j;
         ++cnt;
         visit(j, pf);
     };
     const int nc = D.vcnt[s];
+    if (LPT > 1 && nc <= P.verlet_cap) {
+        const int* __restrict__ nl = D.vnbr + vlist_at(s, 0, P.verlet_cap);
+        const int gbase = (int)(threadIdx.x & 63) - sub;
+        const uint64_t below = (1ull << sub) - 1ull;
+        for (int b = 0; b < nc; b += LPT) {
+            const int k = b + sub;
+            int j = 0;
+            bool ok = false;
+            if (k < nc) {
+                j = __builtin_nontemporal_load(nl + (int64_t)k * kVlistLanes);
+                ok = accept(j, D.posf[(uint32_t)j]);
+            }
+            const uint64_t m = (__ballot(ok) >> gbase) & ((1ull << LPT) - 1ull);
+            if (ok) {
+                const int pos = cnt + __popcll(m & below);
+                if (pos < cap) D.nbr[vlist_at(s, pos, cap)] = j;
+            }
+            cnt += __popcll(m);
+        }
+        return cnt;
+    }
+    if (LPT > 1 && sub != 0) return 0;
     if (nc > P.verlet_cap) {
         const uint32_t key = D.key[s];
         const int cx = (int)(key % (uint32_t)P.nc[0]);
@@ -168,19 +201,21 @@ __device__ __forceinline__ void close_gate(const Params& P) {
 
 // The exact lists of this step for every real target (filter_target), when
 // no pair pass filters the candidates itself.
-template <int DIM>
+// LPT lanes per target (target_slot): 1, or kLanesPerTarget for small runs.
+template <int DIM, int LPT>
 __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
     close_gate(P);
-    const int s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    int sub;
+    const int s = target_slot<LPT>(P, nullptr, 0, sub);
     if (s >= P.n) return;
     const int4 mt = D.meta[s];
     if (mt.w >= P.n_real) return;
     if (mt.y == -1) {
-        D.ncnt[s] = 0;
+        if (sub == 0) D.ncnt[s] = 0;
         return;
     }
-    const int cnt = filter_target<DIM>(P, D, s, D.posh[s], [](int, const float4&) {});
-    filter_done(P, D, s, cnt);
+    const int cnt = filter_target<DIM, LPT>(P, D, s, D.posh[s], [](int, const float4&) {}, sub);
+    if (sub == 0) filter_done(P, D, s, cnt);
 }
 
 // The step boundary of the general path: the correct of step n

@@ -36,6 +36,7 @@ struct Params {
     int external_force, arti_visc, arti_stress, kgc, has_solid;
     int has_t100;       // some particle has |type| == 100 (the general force pass keeps one launch)
     int n_sidx;         // real solid targets (general force pass)
+    int ext_pow;        // a wall (LJ) or fluid-fluid penalty pair force can fire (their pow terms)
     int mixed;          // FP32 pair physics with density stored as rho - rho0 in FP32
     int pair32;         // general path in mixed precision: FP64 state, FP32 pair arithmetic
     int visc_skip_solid;  // a fluid's viscous pair term skips solid neighbours

@@ -226,14 +226,19 @@ __global__ __launch_bounds__(kBlock) void k_bbox(Params P0, Dev D, RunMap M) {
     bbox_block_reduce(mn, mx, P.partial, lb);
 }
 
+// Threads of the one block per run of k_grid: its reduction over the block
+// partials is a chain of dependent loads per thread, so more threads make it
+// shorter.
+constexpr int kGridThreads = 1024;
+
 // Reference grid of this step, asymmetricSearch.f90:49-131. One block per run.
 template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_grid(Params P0, Dev D, RunMap M) {
+__global__ __launch_bounds__(kGridThreads) void k_grid(Params P0, Dev D, RunMap M) {
     const Params& P = BATCH ? M.runs[M.rows[blockIdx.x]] : P0;
-    constexpr int kWaves = kBlock / 64;
+    constexpr int kWaves = kGridThreads / 64;
     __shared__ double wmn[3][kWaves], wmx[3][kWaves];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = threadIdx.x; b < P.nblk; b += kBlock)
+    for (int b = threadIdx.x; b < P.nblk; b += kGridThreads)
         for (int d = 0; d < 3; ++d) {
             mn[d] = fmin(mn[d], P.partial[b * 6 + d]);
             mx[d] = fmax(mx[d], P.partial[b * 6 + 3 + d]);
@@ -1158,8 +1163,10 @@ constexpr int kTgtAll = 0, kTgtFluid = 1, kTgtSolid = 2;
 // The force pass: everything single_step adds into Delta (singleStep.f90:139-144)
 // plus continuity density and the divergences. TGT: target class; SPH: 2
 // compiles the SPH = 2 form alone, 0 takes P.pa_sph; LPT: lanes per target
-// (the solid class takes the listed solid slots).
-template <int DIM, typename R, int TGT, int SPH, int LPT>
+// (the solid class takes the listed solid slots); POW: the wall (LJ) and
+// fluid-fluid penalty forces are compiled in (a run where neither can fire,
+// P.ext_pow == 0, leaves their pow terms out).
+template <int DIM, typename R, int TGT, int SPH, int LPT, bool POW>
 __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
     // mixed precision (R = float) gathers the FP32 pair records; FP64 the
     // FP64 state. Every per-pair value is an R, so the FP64 arithmetic is
@@ -1330,15 +1337,16 @@ __global__ __launch_bounds__(kBlock) void k_force(Params P, Dev D) {
         }
         // external pair forces (externalForce.f90:103-174)
         if (P.external_force) {
-            if (ti > 0 && tj < 0 && (double)g.r < P.lj_r0) {
+            if (POW && ti > 0 && tj < 0 && (double)g.r < P.lj_r0) {
                 R rr = (R)P.lj_r0 / g.r;
                 R fl = (R)P.lj_factor * (pow(rr, (R)P.lj_p1) - pow(rr, (R)P.lj_p2)) / (g.r * g.r);
                 for (int d = 0; d < DIM; ++d) acc[d] += fl * g.dx[d];
             }
             if (ti != tj && ti > 0 && tj > 0) {
                 if (ti < 100 && tj < 100) {
+                    // (without POW the run has one fluid type: never taken)
                     R pe = (hi + hj) / ((R)2 * g.r);
-                    if (pe >= (R)1) {
+                    if (POW && pe >= (R)1) {
                         R fp = (R)P.pen_factor * (pow(pe, (R)P.pen_n1) - pow(pe, (R)P.pen_n2)) / (g.r * g.r);
                         for (int d = 0; d < DIM; ++d) acc[d] += fp * g.dx[d];
                     }
@@ -2140,7 +2148,7 @@ template <int DIM>
 static void launch_fast_head(sphgpu_ctx* c, int mode) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, P, c->D, RunMap{});
     if (P.mixed)
         hipLaunchKernelGGL((k_point_fast<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, mode, RunMap{});
     else
@@ -2187,7 +2195,7 @@ static void fast_body_step(sphgpu_ctx* c, bool timed) {
         hipLaunchKernelGGL((k_correct_predict<DIM, true, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
     else
         hipLaunchKernelGGL((k_correct_predict<DIM, false, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, first, RunMap{}, 0);
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, P, c->D, RunMap{});
     c->time_force = false;
 }
 
@@ -2326,12 +2334,12 @@ static void launch_force_classes(sphgpu_ctx* c) {
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     if (P.n_real <= kSmallRun) {
         const int nbl = (int)(((int64_t)P.n * kLanesPerTarget + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL((k_force<DIM, R, kTgtAll, SPH, kLanesPerTarget>), dim3(nbl), dim3(kBlock), 0, c->stream,
-                           P, c->D);
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtAll, SPH, kLanesPerTarget, true>), dim3(nbl), dim3(kBlock), 0,
+                           c->stream, P, c->D);
         return;
     }
     if (P.has_t100) {
-        hipLaunchKernelGGL((k_force<DIM, R, kTgtAll, SPH, 1>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtAll, SPH, 1, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
         return;
     }
     // the solid targets are few: they take several lanes each, on the side
@@ -2341,11 +2349,14 @@ static void launch_force_classes(sphgpu_ctx* c) {
         const int nbs = (int)(((int64_t)P.n_sidx * kLanesPerTarget + kBlock - 1) / kBlock);
         hipEventRecord(c->fork, c->stream);
         hipStreamWaitEvent(c->side, c->fork, 0);
-        hipLaunchKernelGGL((k_force<DIM, R, kTgtSolid, SPH, kLanesPerTarget>), dim3(nbs), dim3(kBlock), 0, c->side,
-                           P, c->D);
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtSolid, SPH, kLanesPerTarget, true>), dim3(nbs), dim3(kBlock), 0,
+                           c->side, P, c->D);
         hipEventRecord(c->join, c->side);
     }
-    hipLaunchKernelGGL((k_force<DIM, R, kTgtFluid, SPH, 1>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (P.ext_pow)
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtFluid, SPH, 1, true>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    else
+        hipLaunchKernelGGL((k_force<DIM, R, kTgtFluid, SPH, 1, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
     if (solids) hipStreamWaitEvent(c->stream, c->join, 0);
 }
 
@@ -2399,7 +2410,7 @@ static void launch_physics(sphgpu_ctx* c) {
 template <int DIM>
 static void gen_head(sphgpu_ctx* c, int mode) {
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
     hipLaunchKernelGGL(k_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, mode);
 }
 
@@ -2412,7 +2423,13 @@ static void gen_tail(sphgpu_ctx* c, int count_only) {
     hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only,
                        RunMap{});
     if (count_only) return;
-    if (!strain_filters(c)) hipLaunchKernelGGL(k_filter<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    if (strain_filters(c)) {
+    } else if (P.n_real <= kSmallRun) {
+        const int nbl = (int)(((int64_t)P.n * kLanesPerTarget + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL((k_filter<DIM, kLanesPerTarget>), dim3(nbl), dim3(kBlock), 0, c->stream, P, c->D);
+    } else {
+        hipLaunchKernelGGL((k_filter<DIM, 1>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D);
+    }
     launch_physics<DIM>(c);
 }
 
@@ -2427,7 +2444,7 @@ static void launch_derivs(sphgpu_ctx* c, int mode) {
         gen_tail<DIM>(c, 0);
         return;
     }
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, mode);
     launch_physics<DIM>(c);
@@ -2454,7 +2471,7 @@ static void gen_body_step(sphgpu_ctx* c) {
     const int first = c->step == 1;
     gen_tail<DIM>(c, 0);
     hipLaunchKernelGGL(k_correct_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
     hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(std::min(nb, kStrideBlocks)), dim3(kBlock), 0, c->stream,
                        c->P, c->D, RunMap{});
 }
@@ -2648,7 +2665,7 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     HIPCHK(c, hipMemsetAsync(c->D.maxcnt, 0, sizeof(int), c->stream));
     hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
-    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+    hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
     launch_sort<DIM>(c);
     launch_point_build<DIM>(c, 2 | 4);
     HIPCHK(c, hipMemcpyAsync(c->h_err + 1, c->D.maxcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -2755,14 +2772,21 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
         if (type[i] == 100 || type[i] == -100) P.has_t100 = 1;
     }
     {
-        bool multi = false;
-        int first_pos = 0;
+        // pair forces between particles of two positive types need at least
+        // two of them; the fluid-fluid penalty needs two fluid (< 100) types
+        bool multi = false, multi_fluid = false;
+        int first_pos = 0, first_fluid = 0;
         for (int64_t i = 0; i < n; ++i)
             if (type[i] > 0) {
                 if (first_pos == 0) first_pos = type[i];
                 else if (type[i] != first_pos) multi = true;
+                if (type[i] < 100) {
+                    if (first_fluid == 0) first_fluid = type[i];
+                    else if (type[i] != first_fluid) multi_fluid = true;
+                }
             }
         P.pair_ext = cfg.external_force && (cfg.lj_r0 > 0 || multi);
+        P.ext_pow = cfg.external_force && (cfg.lj_r0 > 0 || multi_fluid);
     }
     hr.fast = fast_path_ok(cfg, P, force_general);
     if (P.mixed && !hr.fast) {
@@ -3283,11 +3307,11 @@ extern "C" int sphgpu_export_pairs(sphgpu_ctx* c, int64_t* offsets, int32_t* nbr
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox<false>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
     if (c->dim == 2) {
-        hipLaunchKernelGGL((k_grid<2, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+        hipLaunchKernelGGL((k_grid<2, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
         launch_sort<2>(c);
         launch_point_build<2>(c, 2);
     } else {
-        hipLaunchKernelGGL((k_grid<3, false>), dim3(1), dim3(kBlock), 0, c->stream, c->P, c->D, RunMap{});
+        hipLaunchKernelGGL((k_grid<3, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
         launch_sort<3>(c);
         launch_point_build<3>(c, 2);
     }
