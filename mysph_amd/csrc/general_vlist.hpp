@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
 // and the bounding-box partials of the new positions for the grid of step
 // n + 1. Dummies are static: their EOS and records were written by
 // k_point_gen and do not change. The reference cells follow once the new grid
-// is known (k_refcell).
+// is known (with the keys of the next tail).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_correct_point_gen(Params P, Dev D, int first) {
     const int s = blockIdx.x * kBlock + threadIdx.x;

@@ -315,21 +315,10 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Params P0, Dev D, RunMap 
     }
 }
 
-// Our cell key (clamped into the fixed grid) and the reference cell.
-template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int with_ref, RunMap M) {
-    // with_ref 1: reference cells with the keys; 2 (one run): reference cells
-    // whenever the reference window can exclude pairs, keys only when the
-    // gate is open (the fast path's per-step reference cells ride on this
-    // launch instead of one of their own)
-    const bool keys_on = gate_open(G);
-    const bool ref_on = with_ref == 1 || (!BATCH && with_ref == 2 && P0.grid->need_win);
-    if (!keys_on && !ref_on) return;
-    const int lim = BATCH ? M.nslot : P0.n;
-    for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
-    const Params* Pp = slot_run<BATCH>(P0, M, s);
-    if (!Pp || (BATCH && !gate_open(Gate{Pp->vflag}))) continue;   // only runs that rebuild
-    const Params& P = *Pp;
+// Our cell key of slot s (clamped into the fixed grid; the off cell ncell for
+// a particle out of the run) counted into its cell, and the reference cell.
+template <int DIM>
+__device__ __forceinline__ void key_slot(const Params& P, const Dev& D, int s, bool keys_on, bool ref_on) {
     double4 q = D.posh[s];
     int4 mt = D.meta[s];
     uint32_t key;
@@ -360,6 +349,22 @@ __global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int w
         D.key[s] = key;
         atomicAdd(&D.count[key], 1);
     }
+}
+
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_keys(Params P0, Dev D, Gate G, int with_ref, RunMap M) {
+    // with_ref 1: reference cells with the keys; 2 (one run): reference cells
+    // whenever the reference window can exclude pairs, keys only when the
+    // gate is open (the fast path's per-step reference cells ride on this
+    // launch instead of one of their own)
+    const bool keys_on = gate_open(G);
+    const bool ref_on = with_ref == 1 || (!BATCH && with_ref == 2 && P0.grid->need_win);
+    if (!keys_on && !ref_on) return;
+    const int lim = BATCH ? M.nslot : P0.n;
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < lim; s += gridDim.x * kBlock) {
+        const Params* Pp = slot_run<BATCH>(P0, M, s);
+        if (!Pp || (BATCH && !gate_open(Gate{Pp->vflag}))) continue;   // only runs that rebuild
+        key_slot<DIM>(*Pp, D, s, keys_on, ref_on);
     }
 }
 
@@ -423,19 +428,11 @@ __global__ __launch_bounds__(kBlock) void k_scan2(int* bsum, int nb, Gate G) {
 // LDS so that global loads and stores stay coalesced. With own_prefix the
 // block adds up the raw tile totals of k_scan1 before it (cheap while there
 // are few tiles, and one launch fewer); otherwise bsum was scanned by k_scan2.
-__global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count, int m,
-                                                  const int* bsum, int* start, int own_prefix, Gate G) {
-    if (!gate_open(G)) return;
-    int base_off = 0;
-    if (own_prefix) {
-        int acc = 0;
-        for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) acc += bsum[b];
-        block_exclusive_scan(acc, &base_off);
-    } else {
-        base_off = bsum[blockIdx.x];
-    }
+// start[base + i] = base_off + exclusive prefix of count within the tile of
+// kScanTile entries at base; returns the tile's total.
+__device__ __forceinline__ int scan_tile(const int* __restrict__ count, int m, int base, int base_off,
+                                         int* start) {
     __shared__ int tile[kScanTile];
-    const int base = blockIdx.x * kScanTile;
     for (int k = 0; k < kScanItems; ++k) {
         int i = threadIdx.x + k * kBlock;
         tile[i] = (base + i < m) ? count[base + i] : 0;
@@ -458,6 +455,22 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count,
         int i = threadIdx.x + k * kBlock;
         if (base + i < m) start[base + i] = tile[i];
     }
+    __syncthreads();   // the tile is free for the next call
+    return tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ count, int m,
+                                                  const int* bsum, int* start, int own_prefix, Gate G) {
+    if (!gate_open(G)) return;
+    int base_off = 0;
+    if (own_prefix) {
+        int acc = 0;
+        for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) acc += bsum[b];
+        block_exclusive_scan(acc, &base_off);
+    } else {
+        base_off = bsum[blockIdx.x];
+    }
+    const int tot = scan_tile(count, m, blockIdx.x * kScanTile, base_off, start);
     if (threadIdx.x == 0) {
         if (own_prefix && blockIdx.x == gridDim.x - 1) start[m] = base_off + tot;
         if (!own_prefix && blockIdx.x == 0) start[m] = bsum[gridDim.x];
@@ -490,29 +503,74 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Params P, Dev D, Gate G, Run
 // grid origin) in cell order with the slot in w: the candidate loop of the
 // search streams these 16-byte records. One thread per cell.
 template <bool BATCH>
+__device__ __forceinline__ void sort_cell(const Params& P0, const Dev& D, int c, const RunMap& M) {
+    const int a = D.start[c], b = D.start[c + 1];
+    if (a == b) return;
+    int* perm = D.perm;
+    for (int i = a + 1; i < b; ++i) {
+        int v = perm[i];
+        int j = i - 1;
+        while (j >= a && perm[j] > v) {
+            perm[j + 1] = perm[j];
+            --j;
+        }
+        perm[j + 1] = v;
+    }
+    const Params& P = BATCH ? M.runs[M.runof[perm[a]]] : P0;   // one run per cell
+    for (int k = a; k < b; ++k) {
+        const int j = perm[k];
+        const double4 q = D.posh[j];
+        D.cpos[k] = make_float4((float)(q.x - P.org[0]), (float)(q.y - P.org[1]),
+                                (float)(q.z - P.org[2]), __int_as_float(j));
+    }
+}
+
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_cellsort(Params P0, Dev D, int m, Gate G, RunMap M) {
     if (!gate_open(G)) return;
-    for (int c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) {
-        const int a = D.start[c], b = D.start[c + 1];
-        if (a == b) continue;
-        int* perm = D.perm;
-        for (int i = a + 1; i < b; ++i) {
-            int v = perm[i];
-            int j = i - 1;
-            while (j >= a && perm[j] > v) {
-                perm[j + 1] = perm[j];
-                --j;
-            }
-            perm[j + 1] = v;
-        }
-        const Params& P = BATCH ? M.runs[M.runof[perm[a]]] : P0;   // one run per cell
-        for (int k = a; k < b; ++k) {
-            const int j = perm[k];
-            const double4 q = D.posh[j];
-            D.cpos[k] = make_float4((float)(q.x - P.org[0]), (float)(q.y - P.org[1]),
-                                    (float)(q.z - P.org[2]), __int_as_float(j));
-        }
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < m; c += gridDim.x * kBlock) sort_cell<BATCH>(P0, D, c, M);
+}
+
+// Runs of at most this many slots and cells (cell table entries, the off cell
+// included) sort in one launch of one block, k_sort_small.
+constexpr int kSmallSortSlots = 1 << 16;
+constexpr int kSmallSortCells = 1 << 18;
+
+// The whole cell sort of a small run in one launch: the reference cells (as
+// k_keys) over the launch's blocks, then block 0 alone does the keys, the
+// scans of k_scan1/k_scan3, k_scatter and k_cellsort with its barriers
+// between them. The same tables as the five launches, in one launch that
+// returns at once when no rebuild is due and no reference cells are asked
+// for. Cell order within a cell comes from the per-cell sort, so the tables
+// do not depend on the order of the counting atomics. Each phase of block 0
+// reads what its other threads wrote in the one before: a device-scope fence
+// (the writes and atomics are done, the L1 is dropped) and the block barrier
+// separate them.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_sort_small(Params P, Dev D, Gate G, int with_ref, int m) {
+    const bool keys_on = gate_open(G);
+    const bool ref_on = with_ref == 1 || (with_ref == 2 && P.grid->need_win);
+    if (!keys_on && !ref_on) return;
+    if (ref_on)
+        for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock)
+            key_slot<DIM>(P, D, s, false, true);
+    if (!keys_on || blockIdx.x != 0) return;
+    for (int s = threadIdx.x; s < P.n; s += kBlock) key_slot<DIM>(P, D, s, true, false);
+    __threadfence();
+    __syncthreads();
+    int carry = 0;
+    for (int base = 0; base < m; base += kScanTile) carry += scan_tile(D.count, m, base, carry, D.start);
+    if (threadIdx.x == 0) D.start[m] = carry;
+    __threadfence();
+    __syncthreads();
+    for (int s = threadIdx.x; s < P.n; s += kBlock) {
+        const uint32_t key = D.key[s];
+        const int pos = D.start[key] + atomicSub(&D.count[key], 1) - 1;
+        D.perm[pos] = s;
     }
+    __threadfence();
+    __syncthreads();
+    for (int c = threadIdx.x; c < m; c += kBlock) sort_cell<false>(P, D, c, RunMap{});
 }
 
 template <int DIM>
@@ -2127,6 +2185,11 @@ static void launch_sort(sphgpu_ctx* c, Gate G = Gate{nullptr}, int with_ref = 1)
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const int nbs = std::min(nb, kStrideBlocks);
+    if (P.n <= kSmallSortSlots && P.ncell + 1 <= kSmallSortCells) {
+        hipLaunchKernelGGL(k_sort_small<DIM>, dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, with_ref,
+                           (int)P.ncell + 1);
+        return;
+    }
     hipLaunchKernelGGL((k_keys<DIM, false>), dim3(nbs), dim3(kBlock), 0, c->stream, P, c->D, G, with_ref, RunMap{});
     const int m = (int)P.ncell + 1;
     const int sb = (m + kScanTile - 1) / kScanTile;
@@ -2419,7 +2482,8 @@ static void gen_tail(sphgpu_ctx* c, int count_only) {
     Params& P = c->P;
     const int nb = (int)((P.n + kBlock - 1) / kBlock);
     const Gate G = fast_gate(c);
-    launch_sort<DIM>(c, G, 0);
+    // the reference cells of this step ride on the key launch (with_ref 2)
+    launch_sort<DIM>(c, G, 2);
     hipLaunchKernelGGL((k_verlet_build<DIM, false>), dim3(nb), dim3(kBlock), 0, c->stream, P, c->D, G, count_only,
                        RunMap{});
     if (count_only) return;
@@ -2464,7 +2528,8 @@ static void launch_step(sphgpu_ctx* c) {
 
 // A step of the general path with a candidate list that is followed by
 // another: its tail, the correct fused with the next step's predict, and the
-// next step's reference grid and cells.
+// next step's reference grid (its reference cells come with the next tail's
+// keys).
 template <int DIM>
 static void gen_body_step(sphgpu_ctx* c) {
     const int nb = (int)((c->P.n + kBlock - 1) / kBlock);
@@ -2472,8 +2537,6 @@ static void gen_body_step(sphgpu_ctx* c) {
     gen_tail<DIM>(c, 0);
     hipLaunchKernelGGL(k_correct_point_gen<DIM>, dim3(nb), dim3(kBlock), 0, c->stream, c->P, c->D, first);
     hipLaunchKernelGGL((k_grid<DIM, false>), dim3(1), dim3(kGridThreads), 0, c->stream, c->P, c->D, RunMap{});
-    hipLaunchKernelGGL((k_refcell<DIM, false>), dim3(std::min(nb, kStrideBlocks)), dim3(kBlock), 0, c->stream,
-                       c->P, c->D, RunMap{});
 }
 
 // nsteps steps of the general path. With a candidate list: the head of the
