@@ -358,7 +358,16 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             }
     };
     if constexpr (MIXED) {
-        auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj) {
+        // the window cell of each candidate is read with its records (a load
+        // inside the window branch would hold the wave until every record
+        // load in flight had returned)
+        const bool rc_ld = g.need_win;
+        auto in_window_rc = [&](uint32_t rcj) {
+            bool inwin = true;
+            for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
+            return inwin | !need_win;
+        };
+        auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj, uint32_t rcj) {
             const uint32_t pk = __float_as_uint(pf.w);
             int cxj, cyj, czj;
             unpack_cell<DIM>(pk, cxj, cyj, czj);
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             const R r2t = (int)(pk | pk_i) < 0 ? lof : r2;
             bool ok = r2t < hif;
             if (ok & !(r2t < lof)) ok = exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) >= 0.0;
-            if (need_win & ok) ok = in_window(j);
+            ok = ok & in_window_rc(rcj);
             // every candidate runs the pair arithmetic and a rejected one adds
             // zeros (G = 0): no divergent branch around the sums. vm.w = m / rho;
             // one rsq gives r and 1 / r; both kernel branches are cheap, so
@@ -388,29 +397,51 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             sum_pair(j, dx, dy, dz, r2, r, ok ? (q < (R)1 ? Gn : Gf) : (R)0, vj, thj, thj.w, vj.w,
                      vj.w * thj.x);
         };
-        // software pipeline: the records of candidate k + 1 are in flight
-        // while candidate k is tested and summed (loaded whether or not it
-        // turns out to be a neighbour; indices past the end repeat the last
-        // valid one). The candidate list is read once per pass: nontemporal
-        // loads keep it from evicting neighbour records from L1/L2. Unsigned
-        // slot indices make each record address one shift-add.
+        // software pipeline two candidates deep: the records of candidates
+        // k + 1 and k + 2 are in flight while candidate k is tested and
+        // summed (loaded whether or not they turn out to be neighbours).
+        // Three record sets take turns in a three-candidate body, so no set
+        // is copied (a copy would wait for its loads). The candidate list is
+        // read once per pass: nontemporal loads keep it from evicting
+        // neighbour records from L1/L2. Unsigned slot indices make each
+        // record address one shift-add.
         constexpr int ldn = kVlistLanes;
-        int j = n > 0 ? __builtin_nontemporal_load(nl) : s;
-        int jn = n > 1 ? __builtin_nontemporal_load(nl + ldn) : j;
-        float4 pf = D.posf[(uint32_t)j], vm = D.prf[(uint32_t)j].vm, th = D.prf[(uint32_t)j].th;
-        for (int k = 0; k < n; ++k) {
-            const int jnn = k + 2 < n ? __builtin_nontemporal_load(nl + (int64_t)(k + 2) * ldn) : jn;
-            const float4 pfn = D.posf[(uint32_t)jn], vmn = D.prf[(uint32_t)jn].vm, thn = D.prf[(uint32_t)jn].th;
-            candidate(j, pf, vm, th);
-            j = jn;
-            jn = jnn;
-            pf = pfn;
-            vm = vmn;
-            th = thn;
+        struct Rec {
+            int j;
+            float4 pf, vm, th;
+            uint32_t rc;
+        };
+        auto load = [&](Rec& r, int j) {
+            r.j = j;
+            r.pf = D.posf[(uint32_t)j];
+            r.vm = D.prf[(uint32_t)j].vm;
+            r.th = D.prf[(uint32_t)j].th;
+            r.rc = rc_ld ? D.refcell[(uint32_t)j] : 0u;
+        };
+        // an index past the end reads this particle's own records (never
+        // summed); it depends on no load in flight
+        auto index = [&](int k) { return k < n ? __builtin_nontemporal_load(nl + (int64_t)k * ldn) : s; };
+        const int i0 = index(0), i1 = index(1);
+        int i2 = index(2), i3 = index(3), i4 = index(4);
+        Rec A, B, C;
+        load(A, i0);
+        load(B, i1);
+        for (int k = 0; k < n; k += 3) {
+            const int i5 = index(k + 5), i6 = index(k + 6), i7 = index(k + 7);
+            load(C, i2);
+            candidate(A.j, A.pf, A.vm, A.th, A.rc);
+            load(A, i3);
+            if (k + 1 < n) candidate(B.j, B.pf, B.vm, B.th, B.rc);
+            load(B, i4);
+            if (k + 2 < n) candidate(C.j, C.pf, C.vm, C.th, C.rc);
+            i2 = i5;
+            i3 = i6;
+            i4 = i7;
         }
         if (sweep)
             for_window([&](int j) {
-                candidate(j, D.posf[(uint32_t)j], D.prf[(uint32_t)j].vm, D.prf[(uint32_t)j].th);
+                candidate(j, D.posf[(uint32_t)j], D.prf[(uint32_t)j].vm, D.prf[(uint32_t)j].th,
+                          rc_ld ? D.refcell[(uint32_t)j] : 0u);
             });
     } else {
         auto candidate64 = [&](int j) {
