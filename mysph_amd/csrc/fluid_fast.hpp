@@ -358,91 +358,107 @@ __global__ __launch_bounds__(kBlock) void k_force_fused(Params P0, Dev D, RunMap
             }
     };
     if constexpr (MIXED) {
-        // the window cell of each candidate is read with its records (a load
-        // inside the window branch would hold the wave until every record
-        // load in flight had returned)
-        const bool rc_ld = g.need_win;
+        // the window test |c_j - c_i| <= w of each axis as one unsigned
+        // compare, c_j - c_i + w <= 2w (cell fields are 10 bits); a lane whose
+        // own window cannot exclude a pair takes a bias that passes every cell
+        uint32_t wb[3], wl[3];
+        for (int d = 0; d < 3; ++d) {
+            wb[d] = need_win ? (uint32_t)(wr[d] - rc[d]) : 2048u;
+            wl[d] = need_win ? 2u * (uint32_t)wr[d] : 4096u;
+        }
         auto in_window_rc = [&](uint32_t rcj) {
             bool inwin = true;
-            for (int d = 0; d < DIM; ++d) inwin &= abs(ref_axis(rcj, d) - rc[d]) <= wr[d];
-            return inwin | !need_win;
+            for (int d = 0; d < DIM; ++d) inwin &= (uint32_t)ref_axis(rcj, d) + wb[d] <= wl[d];
+            return inwin;
         };
-        auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj, uint32_t rcj) {
-            const uint32_t pk = __float_as_uint(pf.w);
-            int cxj, cyj, czj;
-            unpack_cell<DIM>(pk, cxj, cyj, czj);
-            const R dx = (pix - pf.x) + (R)(cxi - cxj) * csf;
-            const R dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
-            const R dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
-            const R r2 = dx * dx + dy * dy + dz * dz;
-            // acceptance in bitwise logic (one mask, no nested regions); the
-            // FP64 test runs only where the FP32 distance is within its error
-            // bound of the cutoff or a particle is off the grid (bit 31 of the
-            // cell word): such a pair is tested at r2 = lof, inside the band
-            const R r2t = (int)(pk | pk_i) < 0 ? lof : r2;
-            bool ok = r2t < hif;
-            if (ok & !(r2t < lof)) ok = exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) >= 0.0;
-            ok = ok & in_window_rc(rcj);
-            // every candidate runs the pair arithmetic and a rejected one adds
-            // zeros (G = 0): no divergent branch around the sums. vm.w = m / rho;
-            // one rsq gives r and 1 / r; both kernel branches are cheap, so
-            // select instead of diverging
-            const R rinv = fast_rsq(r2);
-            const R r = r2 * rinv;
-            const R q = r * inv_h;
-            const R t = (R)2 - q;
-            const R Gn = g1 * ((R)-2 + (R)1.5 * q);
-            const R Gf = g2 * t * t * rinv;
-            acc_n += ok ? 1 : 0;
-            sum_pair(j, dx, dy, dz, r2, r, ok ? (q < (R)1 ? Gn : Gf) : (R)0, vj, thj, thj.w, vj.w,
-                     vj.w * thj.x);
+        // the candidate loop in two forms, chosen once per launch: with the
+        // window test, each candidate's window cell read with its records (a
+        // load inside the window branch would hold the wave until every record
+        // load in flight had returned), or, where no particle's window can
+        // exclude a pair, with neither
+        auto run = [&](auto win_tag) {
+            constexpr bool WIN = decltype(win_tag)::value;
+            auto candidate = [&](int j, const float4& pf, const float4& vj, const float4& thj, uint32_t rcj) {
+                const uint32_t pk = __float_as_uint(pf.w);
+                int cxj, cyj, czj;
+                unpack_cell<DIM>(pk, cxj, cyj, czj);
+                const R dx = (pix - pf.x) + (R)(cxi - cxj) * csf;
+                const R dy = (piy - pf.y) + (R)(cyi - cyj) * csf;
+                const R dz = DIM == 3 ? (piz - pf.z) + (R)(czi - czj) * csf : (R)0;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                // acceptance in bitwise logic (one mask, no nested regions); the
+                // FP64 test runs only where the FP32 distance is within its error
+                // bound of the cutoff or a particle is off the grid (bit 31 of the
+                // cell word): such a pair is tested at r2 = lof, inside the band
+                const R r2t = (int)(pk | pk_i) < 0 ? lof : r2;
+                bool ok = r2t < hif;
+                if (ok & !(r2t < lof)) ok = exact_dr<DIM>(qi, D.posh[j], cut, lo2, hi2) >= 0.0;
+                if constexpr (WIN) ok = ok & in_window_rc(rcj);
+                // every candidate runs the pair arithmetic and a rejected one adds
+                // zeros (G = 0): no divergent branch around the sums. vm.w = m / rho;
+                // one rsq gives r and 1 / r; both kernel branches are cheap, so
+                // select instead of diverging
+                const R rinv = fast_rsq(r2);
+                const R r = r2 * rinv;
+                const R q = r * inv_h;
+                const R t = (R)2 - q;
+                const R Gn = g1 * ((R)-2 + (R)1.5 * q);
+                const R Gf = g2 * t * t * rinv;
+                acc_n += ok ? 1 : 0;
+                sum_pair(j, dx, dy, dz, r2, r, ok ? (q < (R)1 ? Gn : Gf) : (R)0, vj, thj, thj.w, vj.w,
+                         vj.w * thj.x);
+            };
+            // software pipeline two candidates deep: the records of candidates
+            // k + 1 and k + 2 are in flight while candidate k is tested and
+            // summed (loaded whether or not they turn out to be neighbours).
+            // Three record sets take turns in a three-candidate body, so no set
+            // is copied (a copy would wait for its loads). The candidate list is
+            // read once per pass: nontemporal loads keep it from evicting
+            // neighbour records from L1/L2. Unsigned slot indices make each
+            // record address one shift-add.
+            constexpr int ldn = kVlistLanes;
+            struct Rec {
+                int j;
+                float4 pf, vm, th;
+                uint32_t rc;
+            };
+            auto load = [&](Rec& r, int j) {
+                r.j = j;
+                r.pf = D.posf[(uint32_t)j];
+                r.vm = D.prf[(uint32_t)j].vm;
+                r.th = D.prf[(uint32_t)j].th;
+                r.rc = WIN ? D.refcell[(uint32_t)j] : 0u;
+            };
+            // an index past the end reads this particle's own records (never
+            // summed); it depends on no load in flight
+            auto index = [&](int k) { return k < n ? __builtin_nontemporal_load(nl + (int64_t)k * ldn) : s; };
+            const int i0 = index(0), i1 = index(1);
+            int i2 = index(2), i3 = index(3), i4 = index(4);
+            Rec A, B, C;
+            load(A, i0);
+            load(B, i1);
+            for (int k = 0; k < n; k += 3) {
+                const int i5 = index(k + 5), i6 = index(k + 6), i7 = index(k + 7);
+                load(C, i2);
+                candidate(A.j, A.pf, A.vm, A.th, A.rc);
+                load(A, i3);
+                if (k + 1 < n) candidate(B.j, B.pf, B.vm, B.th, B.rc);
+                load(B, i4);
+                if (k + 2 < n) candidate(C.j, C.pf, C.vm, C.th, C.rc);
+                i2 = i5;
+                i3 = i6;
+                i4 = i7;
+            }
+            if (sweep)
+                for_window([&](int j) {
+                    candidate(j, D.posf[(uint32_t)j], D.prf[(uint32_t)j].vm, D.prf[(uint32_t)j].th,
+                              WIN ? D.refcell[(uint32_t)j] : 0u);
+                });
         };
-        // software pipeline two candidates deep: the records of candidates
-        // k + 1 and k + 2 are in flight while candidate k is tested and
-        // summed (loaded whether or not they turn out to be neighbours).
-        // Three record sets take turns in a three-candidate body, so no set
-        // is copied (a copy would wait for its loads). The candidate list is
-        // read once per pass: nontemporal loads keep it from evicting
-        // neighbour records from L1/L2. Unsigned slot indices make each
-        // record address one shift-add.
-        constexpr int ldn = kVlistLanes;
-        struct Rec {
-            int j;
-            float4 pf, vm, th;
-            uint32_t rc;
-        };
-        auto load = [&](Rec& r, int j) {
-            r.j = j;
-            r.pf = D.posf[(uint32_t)j];
-            r.vm = D.prf[(uint32_t)j].vm;
-            r.th = D.prf[(uint32_t)j].th;
-            r.rc = rc_ld ? D.refcell[(uint32_t)j] : 0u;
-        };
-        // an index past the end reads this particle's own records (never
-        // summed); it depends on no load in flight
-        auto index = [&](int k) { return k < n ? __builtin_nontemporal_load(nl + (int64_t)k * ldn) : s; };
-        const int i0 = index(0), i1 = index(1);
-        int i2 = index(2), i3 = index(3), i4 = index(4);
-        Rec A, B, C;
-        load(A, i0);
-        load(B, i1);
-        for (int k = 0; k < n; k += 3) {
-            const int i5 = index(k + 5), i6 = index(k + 6), i7 = index(k + 7);
-            load(C, i2);
-            candidate(A.j, A.pf, A.vm, A.th, A.rc);
-            load(A, i3);
-            if (k + 1 < n) candidate(B.j, B.pf, B.vm, B.th, B.rc);
-            load(B, i4);
-            if (k + 2 < n) candidate(C.j, C.pf, C.vm, C.th, C.rc);
-            i2 = i5;
-            i3 = i6;
-            i4 = i7;
-        }
-        if (sweep)
-            for_window([&](int j) {
-                candidate(j, D.posf[(uint32_t)j], D.prf[(uint32_t)j].vm, D.prf[(uint32_t)j].th,
-                          rc_ld ? D.refcell[(uint32_t)j] : 0u);
-            });
+        if (g.need_win)
+            run(std::true_type{});
+        else
+            run(std::false_type{});
     } else {
         auto candidate64 = [&](int j) {
             const double4 qj = D.posh[j];
