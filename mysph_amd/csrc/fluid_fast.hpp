@@ -153,12 +153,12 @@ __global__ __launch_bounds__(kBlock) void k_verlet_build(Params P0, Dev D, Gate 
 template <int DIM>
 __device__ __forceinline__ double exact_dr(const double4& qi, const double4& qj, double cut,
                                            double lo2, double hi2) {
-    const double dx0 = __dsub_rn(qi.x, qj.x);
-    const double dx1 = __dsub_rn(qi.y, qj.y);
-    double dr = __dadd_rn(__dmul_rn(dx0, dx0), __dmul_rn(dx1, dx1));
+    const double dx0 = rn_sub(qi.x, qj.x);
+    const double dx1 = rn_sub(qi.y, qj.y);
+    double dr = rn_add(rn_mul(dx0, dx0), rn_mul(dx1, dx1));
     if (DIM == 3) {
-        const double dx2 = __dsub_rn(qi.z, qj.z);
-        dr = __dadd_rn(dr, __dmul_rn(dx2, dx2));
+        const double dx2 = rn_sub(qi.z, qj.z);
+        dr = rn_add(dr, rn_mul(dx2, dx2));
     }
     if (dr < lo2) return dr;
     if (!(dr < hi2)) return -1.0;
@@ -534,32 +534,32 @@ __global__ __launch_bounds__(kBlock) void k_correct_predict(Params P0, Dev D, in
             if (MIXED) {
                 const float4 dv = D.dvf[s];
                 dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
-                float dr = D.drho_prev[s] + (float)fac * dv.w;   // correct
+                float dr = add_mul(D.drho_prev[s], (float)fac, dv.w);   // correct
                 finite_rho = isfinite(dr);
                 D.drho_prev[s] = dr;
-                dr_pred = dr + (float)h2 * dv.w;                 // predict of the next step
+                dr_pred = add_mul(dr, (float)h2, dv.w);               // predict of the next step
                 D.drho[s] = dr_pred;
             } else {
                 const double4 dv = D.dv[s];
                 dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
-                rho_c = vp.w + fac * dv.w;                      // correct
+                rho_c = add_mul(vp.w, fac, dv.w);                    // correct
                 finite_rho = isfinite(rho_c);
-                D.rho[s] = rho_c + h2 * dv.w;                   // predict of the next step
+                D.rho[s] = add_mul(rho_c, h2, dv.w);                 // predict of the next step
             }
             energy_correct_predict<MIXED>(D, s, fac, h2, dee);
-            v.x = vp.x + fac * dvx;
-            v.y = vp.y + fac * dvy;
-            if (DIM == 3) v.z = vp.z + fac * dvz;
+            v.x = add_mul(vp.x, fac, dvx);
+            v.y = add_mul(vp.y, fac, dvy);
+            if (DIM == 3) v.z = add_mul(vp.z, fac, dvz);
             add_disp<DIM, MIXED>(D, s, first, dt, v);
-            q.x = q.x + dt * v.x;
-            q.y = q.y + dt * v.y;
-            if (DIM == 3) q.z = q.z + dt * v.z;
+            q.x = add_mul(q.x, dt, v.x);
+            q.y = add_mul(q.y, dt, v.y);
+            if (DIM == 3) q.z = add_mul(q.z, dt, v.z);
             D.posh[s] = q;
             // predict of the next step: prev = corrected state, then half kick
             D.vprev[s] = make_double4(v.x, v.y, v.z, MIXED ? vp.w : rho_c);
-            v.x = v.x + h2 * dvx;
-            v.y = v.y + h2 * dvy;
-            if (DIM == 3) v.z = v.z + h2 * dvz;
+            v.x = add_mul(v.x, h2, dvx);
+            v.y = add_mul(v.y, h2, dvy);
+            if (DIM == 3) v.z = add_mul(v.z, h2, dvz);
             if (!MIXED) D.vm[s] = v;
             if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
                 atomicOr(P.err, kErrNonFinite);

@@ -225,6 +225,115 @@ __global__ __launch_bounds__(kBlock) void k_filter(Params P, Dev D) {
 // n + 1. Dummies are static: their EOS and records were written by
 // k_point_gen and do not change. The reference cells follow once the new grid
 // is known (with the keys of the next tail).
+//
+// One real particle's correct and predict read every field once and write it
+// once: the arithmetic of correct_one, predict_one, eos_th and write_rec32 in
+// the same order, without reloading what the launch has just stored (each
+// such reload waits for the store before it).
+template <int DIM>
+__device__ __forceinline__ void correct_predict_gen(const Params& P, const Dev& D, int s, int4 mt, bool first,
+                                                    double4& q) {
+    const double dt = P.dt, h2 = dt / 2;
+    const double fac = first ? dt / 2 : dt;
+    const double4 vp = D.vprev[s];
+    const double4 dv = D.dv[s];
+    const double dee = D.de[s];
+    const double ep = D.eprev[s];
+    double4 v = D.vm[s];
+    double4 ds = D.disp[s];
+    const double p_old = D.p[s];
+    const double c = D.c[s];
+    const double rho_mem = P.summation_density ? D.rho[s] : 0.0;
+    const bool solid = P.has_solid && mt.x > 100;
+    const Material* m = solid ? find_mat(P, mt.x) : nullptr;
+    double Sc[DIM * DIM], dS[DIM * DIM];
+    if (m) {
+        const double* S0 = (first ? D.shear : D.shear_prev) + (int64_t)s * DIM * DIM;
+        const double* dS0 = D.dsdt + (int64_t)s * DIM * DIM;
+        for (int k = 0; k < DIM * DIM; ++k) {
+            Sc[k] = S0[k];
+            dS[k] = dS0[k];
+        }
+    }
+    // correct (timeIntegration.f90:178-252)
+    double rho = P.summation_density ? rho_mem : add_mul(vp.w, fac, dv.w);
+    const bool finite_rho = isfinite(rho);
+    double e = add_mul(ep, fac, dee);
+    if (e < 0) e = 0;
+    v.x = add_mul(vp.x, fac, dv.x);
+    v.y = add_mul(vp.y, fac, dv.y);
+    if (DIM == 3) v.z = add_mul(vp.z, fac, dv.z);
+    if (first) {
+        ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
+    } else {
+        ds.x = add_mul(ds.x, dt, v.x); ds.y = add_mul(ds.y, dt, v.y);
+        if (DIM == 3) ds.z = add_mul(ds.z, dt, v.z);
+    }
+    q.x = add_mul(q.x, dt, v.x);
+    q.y = add_mul(q.y, dt, v.y);
+    if (DIM == 3) q.z = add_mul(q.z, dt, v.z);
+    if (m) {
+        for (int k = 0; k < DIM * DIM; ++k) Sc[k] = add_mul(Sc[k], fac, dS[k]);
+        if (m->sigma_y > 0) {
+            double J2 = 0.0;
+            for (int k = 0; k < DIM * DIM; ++k) J2 = add_mul(J2, Sc[k], Sc[k]);
+            double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
+            for (int k = 0; k < DIM * DIM; ++k) Sc[k] = Sc[k] * sc;
+        }
+    }
+    if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(v.x) || !finite_rho)
+        atomicOr(P.err, kErrNonFinite);
+    // predict of the next step (:137-170): prev = corrected state, half kick
+    D.vprev[s] = make_double4(v.x, v.y, v.z, rho);
+    D.eprev[s] = e;
+    e = add_mul(e, h2, dee);
+    if (e < 0) e = 0;
+    if (!P.summation_density) rho = add_mul(rho, h2, dv.w);
+    v.x = add_mul(v.x, h2, dv.x);
+    v.y = add_mul(v.y, h2, dv.y);
+    if (DIM == 3) v.z = add_mul(v.z, h2, dv.z);
+    D.vm[s] = v;
+    D.rho[s] = rho;
+    D.e[s] = e;
+    D.disp[s] = ds;
+    D.posh[s] = q;
+    if (m) {
+        double* S = D.shear + (int64_t)s * DIM * DIM;
+        double* Sp = D.shear_prev + (int64_t)s * DIM * DIM;
+        double Sn[DIM * DIM];
+        for (int k = 0; k < DIM * DIM; ++k) {
+            Sp[k] = Sc[k];
+            Sn[k] = add_mul(Sc[k], h2, dS[k]);
+        }
+        if (m->sigma_y > 0) {
+            double J2 = 0.0;
+            for (int k = 0; k < DIM * DIM; ++k) J2 = add_mul(J2, Sn[k], Sn[k]);
+            double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
+            for (int k = 0; k < DIM * DIM; ++k) Sn[k] = Sn[k] * sc;
+        }
+        for (int k = 0; k < DIM * DIM; ++k) S[k] = Sn[k];
+    }
+    if (P.pa_sph == 3) D.pold[s] = p_old;
+    // EOS and the pair inputs th = (rho, p/rho^2, c, m/rho)
+    double4 th;
+    if (!P.summation_density) {
+        double p = p_old;
+        if (!(mt.x > 100 || mt.x < -100)) p = eos_p(find_mat(P, mt.x), rho, p);
+        D.p[s] = p;
+        th = make_double4(rho, p / (rho * rho), c, v.w / rho);
+        D.th[s] = th;
+    } else if (P.pair32) {
+        th = D.th[s];
+    }
+    D.posf[s] = cell_offset_record(P, q);
+    if (P.pair32) {
+        PairRecF rec;
+        rec.vm = make_float4((float)v.x, (float)v.y, (float)v.z, (float)th.w);
+        rec.th = make_float4((float)th.x, (float)th.y, (float)th.z, (float)q.w);
+        D.prf[s] = rec;
+    }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_correct_point_gen(Params P, Dev D, int first) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
@@ -234,12 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_point_gen(Params P, Dev D, i
         const int4 mt = D.meta[s];
         double4 q = D.posh[s];
         if (mt.w < P.n_real) {
-            correct_one<DIM, false>(P, D, s, mt, first != 0, q);
-            predict_one<DIM>(P, D, s, mt, false);
-            if (P.pa_sph == 3) D.pold[s] = D.p[s];
-            if (!P.summation_density) eos_th(P, D, s, mt);
-            D.posf[s] = cell_offset_record(P, q);
-            if (P.pair32) write_rec32(D, s);
+            correct_predict_gen<DIM>(P, D, s, mt, first != 0, q);
             moved = moved_half_skin<false>(P, D, s, q);
         }
         mn[0] = mx[0] = q.x; mn[1] = mx[1] = q.y; mn[2] = mx[2] = q.z;

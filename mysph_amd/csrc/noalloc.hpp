@@ -137,26 +137,26 @@ __global__ __launch_bounds__(kBlock) void k_na_grid(NaView v, int n, int nb, int
     double delta[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < 3; ++d) { g.minc[d] = 0.0; g.maxc[d] = 0.0; g.num[d] = 1; }
     for (int d = 0; d < dim; ++d) {
-        delta[d] = __dsub_rn(smx[d][0], smn[d][0]);
-        g.maxc[d] = __dadd_rn(smx[d][0], __dmul_rn(kGmrM1, delta[d]));
-        g.minc[d] = __dsub_rn(smn[d][0], __dmul_rn(kGmrM1, delta[d]));
+        delta[d] = rn_sub(smx[d][0], smn[d][0]);
+        g.maxc[d] = rn_add(smx[d][0], rn_mul(kGmrM1, delta[d]));
+        g.minc[d] = rn_sub(smn[d][0], rn_mul(kGmrM1, delta[d]));
     }
     const double N = (double)n;
     if (dim == 2) {
-        double t = __dsqrt_rn(__ddiv_rn(__dmul_rn(N, delta[0]), __dmul_rn(delta[1], (double)kNpg)));
+        double t = __dsqrt_rn(__ddiv_rn(rn_mul(N, delta[0]), rn_mul(delta[1], (double)kNpg)));
         int c1 = (t >= 0.0 && t < 1e9) ? (int)t + 1 : 0;
         g.num[0] = min(c1, 1000);
-        double t2 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[1]), delta[0]);
+        double t2 = __ddiv_rn(rn_mul((double)g.num[0], delta[1]), delta[0]);
         int c2 = (t2 >= 0.0 && t2 < 1e9) ? (int)t2 + 1 : 0;
         g.num[1] = min(c2, 1000);
     } else {
-        double t = __ddiv_rn(__dmul_rn(__dmul_rn(N, delta[0]), delta[0]),
-                             __dmul_rn(__dmul_rn(delta[1], delta[2]), (double)kNpg));
+        double t = __ddiv_rn(rn_mul(rn_mul(N, delta[0]), delta[0]),
+                             rn_mul(rn_mul(delta[1], delta[2]), (double)kNpg));
         t = pow(t, 1.0 / 3.0);
         int c1 = (t >= 0.0 && t < 1e9) ? (int)t + 1 : 0;
         g.num[0] = min(c1, 1000);
-        double t2 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[1]), delta[0]);
-        double t3 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[2]), delta[0]);
+        double t2 = __ddiv_rn(rn_mul((double)g.num[0], delta[1]), delta[0]);
+        double t3 = __ddiv_rn(rn_mul((double)g.num[0], delta[2]), delta[0]);
         int c2 = (t2 >= 0.0 && t2 < 1e9) ? (int)t2 + 1 : 0;
         int c3 = (t3 >= 0.0 && t3 < 1e9) ? (int)t3 + 1 : 0;
         g.num[1] = min(c2, 1000);
@@ -307,10 +307,10 @@ __device__ __forceinline__ int na_neighbours(const NaView& v, const NaGrid& g, c
         return 0;
     }
     const double hi = h[i];
-    const double cut = __dmul_rn(hi, (double)scale_k);
+    const double cut = rn_mul(hi, (double)scale_k);
     int lo[3] = {1, 1, 1}, hi_c[3] = {1, 1, 1};
     for (int d = 0; d < dim; ++d) {
-        const int w = (int)__dmul_rn(__ddiv_rn(hi, __dsub_rn(g.maxc[d], g.minc[d])), (double)g.num[d]) + 1;
+        const int w = (int)rn_mul(__ddiv_rn(hi, rn_sub(g.maxc[d], g.minc[d])), (double)g.num[d]) + 1;
         hi_c[d] = min(c[d] + w, g.num[d]);
         lo[d] = max(c[d] - w, 1);
     }
@@ -323,11 +323,11 @@ __device__ __forceinline__ int na_neighbours(const NaView& v, const NaGrid& g, c
                 for (int k = b; k < e; ++k) {
                     const int j = v.items[k];
                     if (j == i) continue;
-                    double dx = __dsub_rn(xi[0], x[(int64_t)j * dim]);
-                    double dr = __dmul_rn(dx, dx);
+                    double dx = rn_sub(xi[0], x[(int64_t)j * dim]);
+                    double dr = rn_mul(dx, dx);
                     for (int d = 1; d < dim; ++d) {
-                        dx = __dsub_rn(xi[d], x[(int64_t)j * dim + d]);
-                        dr = __dadd_rn(dr, __dmul_rn(dx, dx));
+                        dx = rn_sub(xi[d], x[(int64_t)j * dim + d]);
+                        dr = rn_add(dr, rn_mul(dx, dx));
                     }
                     if (__dsqrt_rn(dr) < cut) {
                         if (WRITE) out[cnt] = j;

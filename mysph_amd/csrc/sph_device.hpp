@@ -22,6 +22,37 @@ constexpr int kErrNonFinite = 2;
 constexpr int kErrPairCap = 4;
 constexpr int kErrHRange = 8;    // an evolving smoothing length outgrew the planned search range
 
+// IEEE sum, difference and product, each rounded on its own, for arithmetic
+// that must be the reference's (its exact neighbour test, search grid and
+// integrator): the compiler may not fuse these into multiply-adds. HIP's
+// __dadd_rn and __dmul_rn are plain operators in this build, which hipcc
+// contracts into v_fma_f64 like any other a * b + c.
+__device__ __forceinline__ double rn_add(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double rn_sub(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double rn_mul(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+// a + b c with the product and the sum each rounded, as the reference's
+// integrator computes it. The leap-frog updates use it in every launch that
+// applies them, so a fused correct + predict and the separate launches give
+// the same bits.
+__device__ __forceinline__ double add_mul(double a, double b, double c) {
+#pragma clang fp contract(off)
+    return a + b * c;
+}
+__device__ __forceinline__ float add_mul(float a, float b, float c) {
+#pragma clang fp contract(off)
+    return a + b * c;
+}
+
 struct Material {
     int32_t type, eos;
     double rho0, K, gamma, eta, sigma_y;
@@ -261,8 +292,8 @@ __device__ __forceinline__ int ref_locate(double x, double mn, double mx, int nu
         *bad = true;
         return 1;
     }
-    double t = __ddiv_rn(__dsub_rn(x, mn), __dsub_rn(mx, mn));
-    t = __dadd_rn(__dmul_rn(t, (double)num), 1.0);
+    double t = __ddiv_rn(rn_sub(x, mn), rn_sub(mx, mn));
+    t = rn_add(rn_mul(t, (double)num), 1.0);
     return (int)t;
 }
 

@@ -163,8 +163,8 @@ __device__ __forceinline__ void add_disp(const Dev& D, int s, bool first, double
         if (first) {
             ds.x = dt * v.x; ds.y = dt * v.y; ds.z = DIM == 3 ? dt * v.z : 0.0;
         } else {
-            ds.x = ds.x + dt * v.x; ds.y = ds.y + dt * v.y;
-            if (DIM == 3) ds.z = ds.z + dt * v.z;
+            ds.x = add_mul(ds.x, dt, v.x); ds.y = add_mul(ds.y, dt, v.y);
+            if (DIM == 3) ds.z = add_mul(ds.z, dt, v.z);
         }
         D.disp[s] = ds;
     }
@@ -177,17 +177,17 @@ template <bool MIXED>
 __device__ __forceinline__ void energy_correct_predict(const Dev& D, int s, double fac, double h2,
                                                        double dee) {
     if constexpr (MIXED) {
-        float e = D.eprevf[s] + (float)fac * (float)dee;
+        float e = add_mul(D.eprevf[s], (float)fac, (float)dee);
         if (e < 0) e = 0;
         D.eprevf[s] = e;
-        e = e + (float)h2 * (float)dee;
+        e = add_mul(e, (float)h2, (float)dee);
         if (e < 0) e = 0;
         D.ef[s] = e;
     } else {
-        double e = D.eprev[s] + fac * dee;
+        double e = add_mul(D.eprev[s], fac, dee);
         if (e < 0) e = 0;
         D.eprev[s] = e;
-        e = e + h2 * dee;
+        e = add_mul(e, h2, dee);
         if (e < 0) e = 0;
         D.e[s] = e;
     }
@@ -238,11 +238,29 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Params P0, Dev D, RunMap 
     constexpr int kWaves = kGridThreads / 64;
     __shared__ double wmn[3][kWaves], wmx[3][kWaves];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = threadIdx.x; b < P.nblk; b += kGridThreads)
-        for (int d = 0; d < 3; ++d) {
-            mn[d] = fmin(mn[d], P.partial[b * 6 + d]);
-            mx[d] = fmax(mx[d], P.partial[b * 6 + 3 + d]);
+    // the partials of up to kGridUnroll blocks per thread are all loaded
+    // before the first min / max (one memory latency rather than a chain); an
+    // index past the last block reads the last block again, which changes no
+    // min or max
+    constexpr int kGridUnroll = 8;
+    const double2* __restrict__ part = (const double2*)P.partial;
+    for (int b0 = threadIdx.x; b0 < P.nblk; b0 += kGridUnroll * kGridThreads) {
+        double2 v[kGridUnroll][3];
+#pragma unroll
+        for (int u = 0; u < kGridUnroll; ++u) {
+            const int b = min(b0 + u * kGridThreads, P.nblk - 1);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[u][q] = part[b * 3 + q];
         }
+#pragma unroll
+        for (int u = 0; u < kGridUnroll; ++u) {
+            const double w[6] = {v[u][0].x, v[u][0].y, v[u][1].x, v[u][1].y, v[u][2].x, v[u][2].y};
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = fmin(mn[d], w[d]);
+                mx[d] = fmax(mx[d], w[3 + d]);
+            }
+        }
+    }
     // min / max are order-independent: wave shuffles, then the block's waves
     for (int d = 0; d < 3; ++d)
         for (int o = 32; o > 0; o >>= 1) {
@@ -266,26 +284,26 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(Params P0, Dev D, RunMap 
         double delta[3] = {0.0, 0.0, 0.0};
         for (int d = 0; d < 3; ++d) { g.minc[d] = 0.0; g.maxc[d] = 0.0; g.num[d] = 1; }
         for (int d = 0; d < DIM; ++d) {
-            delta[d] = __dsub_rn(smx[d][0], smn[d][0]);
-            g.maxc[d] = __dadd_rn(smx[d][0], __dmul_rn(kGmrM1, delta[d]));
-            g.minc[d] = __dsub_rn(smn[d][0], __dmul_rn(kGmrM1, delta[d]));
+            delta[d] = rn_sub(smx[d][0], smn[d][0]);
+            g.maxc[d] = rn_add(smx[d][0], rn_mul(kGmrM1, delta[d]));
+            g.minc[d] = rn_sub(smn[d][0], rn_mul(kGmrM1, delta[d]));
         }
         double N = (double)P.n;
         if (DIM == 2) {
-            double t = __dsqrt_rn(__ddiv_rn(__dmul_rn(N, delta[0]), __dmul_rn(delta[1], (double)kNpg)));
+            double t = __dsqrt_rn(__ddiv_rn(rn_mul(N, delta[0]), rn_mul(delta[1], (double)kNpg)));
             int c1 = (t >= 0.0 && t < 1e9) ? (int)t + 1 : 0;   // pow(t, 1/2) correctly rounded
             g.num[0] = min(c1, 1000);
-            double t2 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[1]), delta[0]);
+            double t2 = __ddiv_rn(rn_mul((double)g.num[0], delta[1]), delta[0]);
             int c2 = (t2 >= 0.0 && t2 < 1e9) ? (int)t2 + 1 : 0;
             g.num[1] = min(c2, 1000);
         } else {
-            double t = __ddiv_rn(__dmul_rn(__dmul_rn(N, delta[0]), delta[0]),
-                                 __dmul_rn(__dmul_rn(delta[1], delta[2]), (double)kNpg));
+            double t = __ddiv_rn(rn_mul(rn_mul(N, delta[0]), delta[0]),
+                                 rn_mul(rn_mul(delta[1], delta[2]), (double)kNpg));
             t = pow(t, 1.0 / 3.0);
             int c1 = (t >= 0.0 && t < 1e9) ? (int)t + 1 : 0;
             g.num[0] = min(c1, 1000);
-            double t2 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[1]), delta[0]);
-            double t3 = __ddiv_rn(__dmul_rn((double)g.num[0], delta[2]), delta[0]);
+            double t2 = __ddiv_rn(rn_mul((double)g.num[0], delta[1]), delta[0]);
+            double t3 = __ddiv_rn(rn_mul((double)g.num[0], delta[2]), delta[0]);
             int c2 = (t2 >= 0.0 && t2 < 1e9) ? (int)t2 + 1 : 0;
             int c3 = (t3 >= 0.0 && t3 < 1e9) ? (int)t3 + 1 : 0;
             g.num[1] = min(c2, 1000);
@@ -583,12 +601,12 @@ __device__ __forceinline__ void predict_one(const Params& P, const Dev& D, int s
     if (first) return;
     double h2 = P.dt / 2;
     double4 dv = D.dv[s];
-    e = e + h2 * D.de[s];
+    e = add_mul(e, h2, D.de[s]);
     if (e < 0) e = 0;
-    if (!P.summation_density) rho = rho + h2 * dv.w;
-    v.x = v.x + h2 * dv.x;
-    v.y = v.y + h2 * dv.y;
-    if (DIM == 3) v.z = v.z + h2 * dv.z;
+    if (!P.summation_density) rho = add_mul(rho, h2, dv.w);
+    v.x = add_mul(v.x, h2, dv.x);
+    v.y = add_mul(v.y, h2, dv.y);
+    if (DIM == 3) v.z = add_mul(v.z, h2, dv.z);
     D.vm[s] = v;
     D.rho[s] = rho;
     D.e[s] = e;
@@ -601,10 +619,10 @@ __device__ __forceinline__ void predict_one(const Params& P, const Dev& D, int s
         double J2 = 0.0;
         for (int q = 0; q < DIM * DIM; ++q) {
             Sp[q] = S[q];
-            S[q] = S[q] + h2 * dS[q];
+            S[q] = add_mul(S[q], h2, dS[q]);
         }
         if (m->sigma_y > 0) {
-            for (int q = 0; q < DIM * DIM; ++q) J2 += S[q] * S[q];
+            for (int q = 0; q < DIM * DIM; ++q) J2 = add_mul(J2, S[q], S[q]);
             double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
             for (int q = 0; q < DIM * DIM; ++q) S[q] = S[q] * sc;
         }
@@ -634,12 +652,12 @@ __device__ __forceinline__ void predict_mixed(const Params& P, const Dev& D, int
     if (first) return;
     const double h2 = P.dt / 2;
     float4 dv = D.dvf[s];
-    e = e + (float)h2 * D.def[s];
+    e = add_mul(e, (float)h2, D.def[s]);
     if (e < 0) e = 0;
-    dr = dr + (float)h2 * dv.w;
-    v.x = v.x + h2 * (double)dv.x;
-    v.y = v.y + h2 * (double)dv.y;
-    if (DIM == 3) v.z = v.z + h2 * (double)dv.z;
+    dr = add_mul(dr, (float)h2, dv.w);
+    v.x = add_mul(v.x, h2, (double)dv.x);
+    v.y = add_mul(v.y, h2, (double)dv.y);
+    if (DIM == 3) v.z = add_mul(v.z, h2, (double)dv.z);
     D.vm[s] = v;
     D.drho[s] = dr;
     D.ef[s] = e;
@@ -822,12 +840,12 @@ __global__ __launch_bounds__(kBlock) void k_point_build(Params P, Dev D, int mod
         if (!(r2 < lo)) {
             // rounding band: the reference's test, asymmetricSearch.f90:227-237
             const double4 qj = D.posh[j];
-            const double dx0 = __dsub_rn(qi.x, qj.x);
-            const double dx1 = __dsub_rn(qi.y, qj.y);
-            double dr = __dadd_rn(__dmul_rn(dx0, dx0), __dmul_rn(dx1, dx1));
+            const double dx0 = rn_sub(qi.x, qj.x);
+            const double dx1 = rn_sub(qi.y, qj.y);
+            double dr = rn_add(rn_mul(dx0, dx0), rn_mul(dx1, dx1));
             if (DIM == 3) {
-                const double dx2 = __dsub_rn(qi.z, qj.z);
-                dr = __dadd_rn(dr, __dmul_rn(dx2, dx2));
+                const double dx2 = rn_sub(qi.z, qj.z);
+                dr = rn_add(dr, rn_mul(dx2, dx2));
             }
             if (!(__dsqrt_rn(dr) < cut)) return;
         }
@@ -1644,33 +1662,33 @@ __device__ __forceinline__ void correct_one(const Params& P, const Dev& D, int s
     if (MIXED) {
         float4 dv = D.dvf[s];
         dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.def[s];
-        float dr = D.drho_prev[s] + (float)fac * dv.w;
+        float dr = add_mul(D.drho_prev[s], (float)fac, dv.w);
         D.drho[s] = dr;
         finite_rho = isfinite(dr);
     } else {
         double4 dv = D.dv[s];
         dvx = dv.x; dvy = dv.y; dvz = dv.z; dee = D.de[s];
-        if (!P.summation_density) D.rho[s] = vp.w + fac * dv.w;
+        if (!P.summation_density) D.rho[s] = add_mul(vp.w, fac, dv.w);
         finite_rho = isfinite(D.rho[s]);
     }
     if (MIXED) {
-        float e = D.eprevf[s] + (float)fac * (float)dee;
+        float e = add_mul(D.eprevf[s], (float)fac, (float)dee);
         if (e < 0) e = 0;
         D.ef[s] = e;
     } else {
-        double e = D.eprev[s] + fac * dee;
+        double e = add_mul(D.eprev[s], fac, dee);
         if (e < 0) e = 0;
         D.e[s] = e;
     }
     double4 v = D.vm[s];
-    v.x = vp.x + fac * dvx;
-    v.y = vp.y + fac * dvy;
-    if (DIM == 3) v.z = vp.z + fac * dvz;
+    v.x = add_mul(vp.x, fac, dvx);
+    v.y = add_mul(vp.y, fac, dvy);
+    if (DIM == 3) v.z = add_mul(vp.z, fac, dvz);
     D.vm[s] = v;
     add_disp<DIM, MIXED>(D, s, first, dt, v);
-    q.x = q.x + dt * v.x;
-    q.y = q.y + dt * v.y;
-    if (DIM == 3) q.z = q.z + dt * v.z;
+    q.x = add_mul(q.x, dt, v.x);
+    q.y = add_mul(q.y, dt, v.y);
+    if (DIM == 3) q.z = add_mul(q.z, dt, v.z);
     D.posh[s] = q;
     if (!MIXED && P.has_solid && mt.x > 100) {
         const Material* m = find_mat(P, mt.x);
@@ -1678,10 +1696,10 @@ __device__ __forceinline__ void correct_one(const Params& P, const Dev& D, int s
             double* S = D.shear + (int64_t)s * DIM * DIM;
             const double* Sp = D.shear_prev + (int64_t)s * DIM * DIM;
             const double* dS = D.dsdt + (int64_t)s * DIM * DIM;
-            for (int k = 0; k < DIM * DIM; ++k) S[k] = (first ? S[k] : Sp[k]) + fac * dS[k];
+            for (int k = 0; k < DIM * DIM; ++k) S[k] = add_mul(first ? S[k] : Sp[k], fac, dS[k]);
             if (m->sigma_y > 0) {
                 double J2 = 0.0;
-                for (int k = 0; k < DIM * DIM; ++k) J2 += S[k] * S[k];
+                for (int k = 0; k < DIM * DIM; ++k) J2 = add_mul(J2, S[k], S[k]);
                 double sc = fmin(1.0, sqrt(((m->sigma_y * m->sigma_y) / 3) / J2));
                 for (int k = 0; k < DIM * DIM; ++k) S[k] = S[k] * sc;
             }
