@@ -28,3 +28,62 @@ def sorted_pairs(off, nbr):
     nbr = np.asarray(nbr, dtype=np.int64)
     tgt = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off))
     return np.sort((tgt << 32) | nbr)
+
+
+def _fma(a: float, b: float, c: float) -> float:
+    """a * b + c with one rounding (exact rational arithmetic, then the
+    correctly rounded conversion to double)."""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def squared_distance(dx, fused=None):
+    """The reference's squared distance of one pair (asymmetricSearch.f90:
+    209-237): each square and each partial sum rounded on its own, in axis
+    order. fused = "x": dx0^2 fused into the sum with dx1^2, then dx2^2 fused
+    into that; fused = "y": dx1^2 fused into dx0^2 instead (the two forms a
+    compiler that contracts a * a + b builds) -- the mistakes the tests must
+    be able to see."""
+    if fused is None:
+        dr = dx[0] * dx[0] + dx[1] * dx[1]
+        for d in dx[2:]:
+            dr = dr + d * d
+        return dr
+    dr = _fma(dx[0], dx[0], dx[1] * dx[1]) if fused == "x" else _fma(dx[1], dx[1], dx[0] * dx[0])
+    for d in dx[2:]:
+        dr = _fma(d, d, dr)
+    return dr
+
+
+def rounding_boundary_set(dim: int, npairs: int = 48, seed: int = 7):
+    """Isolated particle pairs whose acceptance sqrt(dr) < 2 h flips when the
+    squared distance is formed with a fused multiply-add: for each pair the
+    cutoff 2 h is the larger of the two rounded square roots, so the
+    reference's rounding and a fused one decide it differently. Pairs sit on
+    a lattice 8 cutoffs apart (no pair sees another). Returns (x[n, dim],
+    h[n], accept[npairs]) with accept the reference's decision."""
+    import math
+    rng = np.random.default_rng(seed)
+    xs, hs, acc = [], [], []
+    k = 0
+    while len(acc) < npairs:
+        k += 1
+        side = int(np.ceil(npairs ** (1.0 / dim))) + 1
+        cell = np.array(np.unravel_index(len(acc), (side,) * dim), dtype=np.float64) * 8.0
+        qi = cell + rng.uniform(-0.5, 0.5, dim)
+        u = rng.normal(size=dim)
+        qj = qi + u / np.linalg.norm(u) * rng.uniform(0.9, 1.1)
+        dx = [float(a - b) for a, b in zip(qi, qj)]
+        ref = squared_distance(dx)
+        for f in ("x", "y"):
+            fu = squared_distance(dx, f)
+            s_ref, s_fu = math.sqrt(ref), math.sqrt(fu)
+            if s_ref != s_fu:
+                cut = max(s_ref, s_fu)
+                xs += [qi, qj]
+                hs += [cut / 2, cut / 2]
+                acc.append(s_ref < cut)
+                break
+        if k > 200000:
+            raise RuntimeError("no discriminating pairs found")
+    return np.array(xs), np.array(hs), np.array(acc)

@@ -404,3 +404,28 @@ def test_dam_break_3d_1000_steps_fp64_and_mixed():
         e_x = rel_err(sim.get("x")[:nt], orc["x"][:nt])
         print(f"3D mixed 1000 steps: rho {e_rho:.2e} x {e_x:.2e}")
         assert e_rho <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [C.FP64, C.MIXED])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_exact_test_at_rounding_boundary(dim, precision):
+    """Pairs whose acceptance a fused multiply-add in the squared distance
+    would flip (tests.parity.rounding_boundary_set): the exported exact list
+    and the fused force pass keep exactly the oracle's pairs."""
+    from tests.parity import rounding_boundary_set
+    x, h, accept = rounding_boundary_set(dim)
+    n = len(h)
+    case = dict(dim=dim, n_real=n, n_dummy=0, x=x, h=h, mass=np.ones(n), rho=np.full(n, 1000.0),
+                type=np.full(n, 2, dtype=np.int32))
+    cfg = C.base_config(dim)
+    cfg.add_material(2, C.EOS_TAIT, 1000.0, C.tait_B(10.0), 7.0)
+    cfg.precision = precision
+    case["config"] = cfg
+    o = oracle_pairs(case)
+    assert o.size == 2 * int(accept.sum())
+    print(f"rounding-boundary pairs dim {dim} precision {precision}: {accept.sum()} of {len(accept)} kept",
+          flush=True)
+    with Simulation(case, cfg) as sim:
+        assert np.array_equal(gpu_pairs(sim), o)
+        sim.derivatives()
+        assert sim.neighbour_stats()["pairs"] == o.size

@@ -286,3 +286,30 @@ def test_flat_bounding_box_is_an_error():
     x = np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]])
     with pytest.raises(RuntimeError):
         oracle.bggs(2, 1, 3, x, np.full(3, 0.5))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_exact_test_rounds_each_square_and_sum(dim):
+    """The oracle's BGGS acceptance (asymmetricSearch.f90:209-237) forms the
+    squared distance with every square and partial sum rounded on its own: on
+    pairs where a fused multiply-add would flip the decision (the cutoff is
+    the larger of the two rounded square roots), it keeps exactly the pairs
+    the separately rounded form accepts, and rejects the others."""
+    from tests.parity import rounding_boundary_set, sorted_pairs, squared_distance
+    x, h, accept = rounding_boundary_set(dim)
+    n = len(h)
+    assert 0 < accept.sum() < len(accept)
+    off, nbr, _, _, _ = oracle.bggs(dim, 1, n, x, h)
+    got = sorted_pairs(off, nbr)
+    want = []
+    for k, a in enumerate(accept):
+        if a:
+            want += [((2 * k) << 32) | (2 * k + 1), ((2 * k + 1) << 32) | (2 * k)]
+    assert np.array_equal(got, np.sort(np.array(want, dtype=np.int64)))
+    # a fused evaluation decides every pair of the set the other way
+    for k, a in enumerate(accept):
+        dx = [float(v) for v in x[2 * k] - x[2 * k + 1]]
+        cut = 2.0 * h[2 * k]
+        fused = [math.sqrt(squared_distance(dx, f)) < cut for f in ("x", "y")]
+        assert (math.sqrt(squared_distance(dx)) < cut) == a
+        assert any(fz != a for fz in fused)
