@@ -87,3 +87,27 @@ def rounding_boundary_set(dim: int, npairs: int = 48, seed: int = 7):
         if k > 200000:
             raise RuntimeError("no discriminating pairs found")
     return np.array(xs), np.array(hs), np.array(acc)
+
+
+# real(1.2_4, 8) - 1 (asymmetricSearch.f90:21): the grid's margin factor
+GRID_MARGIN = 0.2000000476837158203125
+
+
+def grid_rounding_cloud(dim: int, n: int = 64, seed: int = 3):
+    """A seeded point cloud whose background-grid bounds (asymmetricSearch.f90:
+    49-131, maxc = max + m (max - min), minc = min - m (max - min)) come out
+    differently when the margin product is fused into the sum: returns x[n,
+    dim] with at least one such axis for maxc and one for minc."""
+    rng = np.random.default_rng(seed)
+    for _ in range(100000):
+        x = rng.uniform(-1.0, 1.0, (n, dim)) * rng.uniform(0.5, 3.0, dim) + rng.uniform(-5, 5, dim)
+        mn, mx = x.min(axis=0), x.max(axis=0)
+        up = lo = False
+        for d in range(dim):
+            delta = float(mx[d] - mn[d])
+            m = GRID_MARGIN * delta
+            up |= float(mx[d]) + m != _fma(GRID_MARGIN, delta, float(mx[d]))
+            lo |= float(mn[d]) - m != _fma(-GRID_MARGIN, delta, float(mn[d]))
+        if up and lo:
+            return x
+    raise RuntimeError("no cloud found")

@@ -429,3 +429,23 @@ def test_exact_test_at_rounding_boundary(dim, precision):
         assert np.array_equal(gpu_pairs(sim), o)
         sim.derivatives()
         assert sim.neighbour_stats()["pairs"] == o.size
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_search_grid_at_rounding_boundary(dim):
+    """A point cloud whose reference grid bounds a fused multiply-add in
+    max + m (max - min) would change (tests.parity.grid_rounding_cloud): the
+    device's grid equals the oracle's bit for bit."""
+    from tests.parity import grid_rounding_cloud
+    x = grid_rounding_cloud(dim)
+    n = len(x)
+    h = np.full(n, 0.05)
+    case = dict(dim=dim, n_real=n, n_dummy=0, x=x, h=h, mass=np.ones(n), rho=np.full(n, 1000.0),
+                type=np.full(n, 2, dtype=np.int32))
+    cfg = C.base_config(dim)
+    cfg.add_material(2, C.EOS_TAIT, 1000.0, C.tait_B(10.0), 7.0)
+    case["config"] = cfg
+    _, _, _, _, og = oracle.bggs(dim, 1, n, x, h)
+    with Simulation(case, cfg) as sim:
+        assert np.array_equal(gpu_pairs(sim), oracle_pairs(case))
+        assert np.array_equal(sim.grid_info(), og)

@@ -313,3 +313,20 @@ def test_exact_test_rounds_each_square_and_sum(dim):
         fused = [math.sqrt(squared_distance(dx, f)) < cut for f in ("x", "y")]
         assert (math.sqrt(squared_distance(dx)) < cut) == a
         assert any(fz != a for fz in fused)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_search_grid_rounds_margin_product_and_sum(dim):
+    """The oracle's background grid bounds (asymmetricSearch.f90:49-131) round
+    the margin product and the sum separately: on a cloud where a fused
+    multiply-add changes them, the oracle's bounds are the separately rounded
+    ones."""
+    from tests.parity import GRID_MARGIN, grid_rounding_cloud
+    x = grid_rounding_cloud(dim)
+    n = len(x)
+    _, _, _, _, g = oracle.bggs(dim, 1, n, x, np.full(n, 0.05))
+    mn, mx = x.min(axis=0), x.max(axis=0)
+    for d in range(dim):
+        delta = float(mx[d] - mn[d])
+        assert g[d] == float(mn[d]) - GRID_MARGIN * delta
+        assert g[3 + d] == float(mx[d]) + GRID_MARGIN * delta

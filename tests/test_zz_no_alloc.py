@@ -156,3 +156,19 @@ def test_noalloc_reports_capacity_and_bad_arguments(hip):
     assert b"holds" in L.sphgpu_last_error(None)
     assert L.sphgpu_pairs_noalloc(4, 1, 10, 20, null, null, null, null, null, 0,
                                   ctypes.byref(cnt), None) == -6          # E_ARG
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_noalloc_exact_test_at_rounding_boundary(hip, dim):
+    """Pairs whose acceptance a fused multiply-add in the squared distance
+    would flip (tests.parity.rounding_boundary_set): the no-alloc export keeps
+    exactly the oracle's pairs."""
+    import oracle
+    from tests.parity import rounding_boundary_set, sorted_pairs
+    x, h, accept = rounding_boundary_set(dim)
+    n = len(h)
+    o_off, o_nbr, _, _, _ = oracle.bggs(dim, 1, n, x, h)
+    rc, need, off, ids = noalloc_pairs(hip, dim, 1, n, x, h, None)
+    assert rc == 0, _lib().sphgpu_last_error(None)
+    assert need == 2 * int(accept.sum())
+    assert np.array_equal(sorted_pairs(off, ids), sorted_pairs(o_off, o_nbr))
