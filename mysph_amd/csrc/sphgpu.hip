@@ -2133,17 +2133,85 @@ static Tuning env_tuning() {
 
 constexpr double kSleHeadroom = 1.5;
 
+// f(lo, hi) over [0, n) split across up to 16 host threads (one thread for
+// small n); each call owns its range, so the result does not depend on the
+// split.
+template <typename F>
+static void host_parallel(int64_t n, F&& f) {
+    const int64_t t = std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < (1 << 16) || t == 1) {
+        f((int64_t)0, n);
+        return;
+    }
+    const int64_t chunk = (n + t - 1) / t;
+    std::vector<std::thread> pool;
+    for (int64_t k = 1; k < t; ++k) {
+        const int64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) continue;
+        try {
+            pool.emplace_back([&f, lo, hi] { f(lo, hi); });
+        } catch (...) {   // no thread to be had: this range runs here
+            f(lo, hi);
+        }
+    }
+    f((int64_t)0, std::min(n, chunk));
+    for (auto& th : pool) th.join();
+}
+
+// host_parallel with each range's number: f(r, lo, hi) for the ranges r =
+// 0..R-1 of its split; returns R.
+template <typename F>
+static int64_t host_ranges(int64_t n, F&& f) {
+    if (n <= 0) return 0;
+    const int64_t t = n < (1 << 16) ? 1 : std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    const int64_t chunk = (n + t - 1) / t;
+    host_parallel(n, [&](int64_t lo, int64_t hi) { f(lo / chunk, lo, hi); });
+    return (n + chunk - 1) / chunk;
+}
+
+// f(k) for k = 0..m-1, each on a host thread of its own (the calling thread
+// takes k = 0, and any k whose thread cannot be created).
+template <typename F>
+static void host_tasks(int64_t m, F&& f) {
+    std::vector<std::thread> pool;
+    for (int64_t k = 1; k < m; ++k) {
+        try {
+            pool.emplace_back([&f, k] { f(k); });
+        } catch (...) {
+            f(k);
+        }
+    }
+    if (m > 0) f(0);
+    for (auto& th : pool) th.join();
+}
+
 static void plan_grid_params(Params& P, int dim, bool verlet, int64_t n, const double* x,
                              const double* h, Tuning tune, double h_next_max = 0.0) {
     double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, hmax = 0;
     for (int d = 0; d < dim; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
-    for (int64_t i = 0; i < n; ++i) {
-        for (int d = 0; d < dim; ++d) {
-            double v = x[i * dim + d];
-            if (v < mn[d]) mn[d] = v;
-            if (v > mx[d]) mx[d] = v;
+    {
+        // per-range extremes, merged (min and max do not depend on the order)
+        std::vector<double> ext(16 * 7, 0.0);
+        const int64_t nr = host_ranges(n, [&](int64_t r, int64_t lo, int64_t hi) {
+            double a[3] = {INFINITY, INFINITY, INFINITY}, b[3] = {-INFINITY, -INFINITY, -INFINITY}, hm = 0;
+            for (int64_t i = lo; i < hi; ++i) {
+                for (int d = 0; d < dim; ++d) {
+                    double v = x[i * dim + d];
+                    if (v < a[d]) a[d] = v;
+                    if (v > b[d]) b[d] = v;
+                }
+                hm = std::max(hm, h[i]);
+            }
+            for (int d = 0; d < 3; ++d) { ext[r * 7 + d] = a[d]; ext[r * 7 + 3 + d] = b[d]; }
+            ext[r * 7 + 6] = hm;
+        });
+        for (int64_t r = 0; r < nr; ++r) {
+            for (int d = 0; d < dim; ++d) {
+                mn[d] = std::min(mn[d], ext[r * 7 + d]);
+                mx[d] = std::max(mx[d], ext[r * 7 + 3 + d]);
+            }
+            hmax = std::max(hmax, ext[r * 7 + 6]);
         }
-        hmax = std::max(hmax, h[i]);
     }
     // an evolving h gets headroom: the grid, the window test and the
     // neighbour capacity hold for h up to kSleHeadroom times the larger of
@@ -2802,30 +2870,6 @@ static int size_neighbour_list(sphgpu_ctx* c) {
     return 0;
 }
 
-// f(lo, hi) over [0, n) split across up to 16 host threads (one thread for
-// small n); each call owns its range, so the result does not depend on the
-// split.
-template <typename F>
-static void host_parallel(int64_t n, F&& f) {
-    const int64_t t = std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
-    if (n < (1 << 16) || t == 1) {
-        f((int64_t)0, n);
-        return;
-    }
-    const int64_t chunk = (n + t - 1) / t;
-    std::vector<std::thread> pool;
-    for (int64_t k = 1; k < t; ++k) {
-        const int64_t lo = k * chunk, hi = std::min(n, lo + chunk);
-        if (lo >= hi) continue;
-        try {
-            pool.emplace_back([&f, lo, hi] { f(lo, hi); });
-        } catch (...) {   // no thread to be had: this range runs here
-            f(lo, hi);
-        }
-    }
-    f((int64_t)0, std::min(n, chunk));
-    for (auto& th : pool) th.join();
-}
 
 static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy, const double* x,
                        const double* v, const double* mass, const double* rho, const double* p,
@@ -2846,28 +2890,53 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     hr.n_dummy = n_dummy;
     Params& P = hr.P;
     fill_params_cfg(P, cfg, n, n_real);
-    P.has_solid = 0;
-    P.has_t100 = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        if (type[i] > 100 || type[i] < -100) P.has_solid = 1;
-        if (type[i] == 100 || type[i] == -100) P.has_t100 = 1;
-    }
     {
-        // pair forces between particles of two positive types need at least
-        // two of them; the fluid-fluid penalty needs two fluid (< 100) types
-        bool multi = false, multi_fluid = false;
-        int first_pos = 0, first_fluid = 0;
-        for (int64_t i = 0; i < n; ++i)
-            if (type[i] > 0) {
-                if (first_pos == 0) first_pos = type[i];
-                else if (type[i] != first_pos) multi = true;
-                if (type[i] < 100) {
-                    if (first_fluid == 0) first_fluid = type[i];
-                    else if (type[i] != first_fluid) multi_fluid = true;
+        // solid and |type| == 100 particles; pair forces between particles of
+        // two positive types need at least two of them, the fluid-fluid
+        // penalty two fluid (< 100) types. Each range of particles reports the
+        // first positive (fluid) type it meets and whether another follows.
+        struct TypeScan {
+            bool solid = false, t100 = false, multi = false, multi_fluid = false;
+            int first_pos = 0, first_fluid = 0;
+        };
+        std::vector<TypeScan> ts(16);
+        const int64_t nr = host_ranges(n, [&](int64_t r, int64_t lo, int64_t hi) {
+            TypeScan a;
+            for (int64_t i = lo; i < hi; ++i) {
+                const int t = type[i];
+                a.solid |= t > 100 || t < -100;
+                a.t100 |= t == 100 || t == -100;
+                if (t > 0) {
+                    if (a.first_pos == 0) a.first_pos = t;
+                    else if (t != a.first_pos) a.multi = true;
+                    if (t < 100) {
+                        if (a.first_fluid == 0) a.first_fluid = t;
+                        else if (t != a.first_fluid) a.multi_fluid = true;
+                    }
                 }
             }
-        P.pair_ext = cfg.external_force && (cfg.lj_r0 > 0 || multi);
-        P.ext_pow = cfg.external_force && (cfg.lj_r0 > 0 || multi_fluid);
+            ts[r] = a;
+        });
+        TypeScan all;
+        for (int64_t r = 0; r < nr; ++r) {
+            const TypeScan& b = ts[r];
+            all.solid |= b.solid;
+            all.t100 |= b.t100;
+            all.multi |= b.multi;
+            all.multi_fluid |= b.multi_fluid;
+            if (b.first_pos) {
+                if (!all.first_pos) all.first_pos = b.first_pos;
+                else if (b.first_pos != all.first_pos) all.multi = true;
+            }
+            if (b.first_fluid) {
+                if (!all.first_fluid) all.first_fluid = b.first_fluid;
+                else if (b.first_fluid != all.first_fluid) all.multi_fluid = true;
+            }
+        }
+        P.has_solid = all.solid ? 1 : 0;
+        P.has_t100 = all.t100 ? 1 : 0;
+        P.pair_ext = cfg.external_force && (cfg.lj_r0 > 0 || all.multi);
+        P.ext_pow = cfg.external_force && (cfg.lj_r0 > 0 || all.multi_fluid);
     }
     hr.fast = fast_path_ok(cfg, P, force_general);
     if (P.mixed && !hr.fast) {
@@ -2883,8 +2952,15 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
                       "(the reference divides by a zero matrix there)";
                 return SPHGPU_E_BAD_CONFIG;
             }
-    for (int64_t i = 0; i < n * dim; ++i)
-        if (!std::isfinite(x[i])) { why = "non-finite position"; return SPHGPU_E_NONFINITE; }
+    {
+        std::vector<char> bad(16, 0);
+        host_ranges(n * dim, [&](int64_t r, int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i)
+                if (!std::isfinite(x[i])) { bad[r] = 1; break; }
+        });
+        for (char b : bad)
+            if (b) { why = "non-finite position"; return SPHGPU_E_NONFINITE; }
+    }
     double h_next_max = 0.0;
     if (P.sle == 1)   // hsml.f90:22-25, with the uploaded density
         for (int64_t i = 0; i < n_real; ++i)
@@ -2916,7 +2992,17 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
     std::vector<uint32_t>& start = hr.sort_start;
     std::vector<int64_t>& order = hr.order;
     key.resize(n);
-    host_parallel(n, [&](int64_t lo, int64_t hi) {
+    // A stable counting sort in O(n) on the host threads. The key range is
+    // split among O owners. Each range of particle indices computes its keys
+    // and lists its indices by owner (in index order); each owner then
+    // counts, scans and places its keys' particles, taking the ranges in
+    // index order, so a cell keeps ascending indices.
+    const int64_t O = n < (1 << 16) ? 1 : std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    const int64_t kchunk = (nkeys + O - 1) / O;
+    std::vector<std::vector<int32_t>> byown(16);
+    std::vector<int64_t> ownoff(16 * 17, 0);
+    const int64_t R = host_ranges(n, [&](int64_t r, int64_t lo, int64_t hi) {
+        int64_t* off = &ownoff[r * 17];
         for (int64_t i = lo; i < hi; ++i) {
             int64_t cc[3] = {0, 0, 0};
             for (int d = 0; d < dim; ++d) {
@@ -2927,26 +3013,45 @@ static int prepare_run(const sphgpu_config& cfg, int64_t n_real, int64_t n_dummy
             const int64_t tile = (cc[2] / te[2] * tn[1] + cc[1] / te[1]) * tn[0] + cc[0] / te[0];
             const int64_t local = (cc[2] % te[2] * te[1] + cc[1] % te[1]) * te[0] + cc[0] % te[0];
             key[i] = (uint32_t)(tile * tcells + local);
+            ++off[key[i] / kchunk + 1];
         }
+        for (int64_t o = 0; o < O; ++o) off[o + 1] += off[o];
+        std::vector<int32_t>& list = byown[r];
+        list.resize(hi - lo);
+        std::vector<int64_t> pos(off, off + O);
+        for (int64_t i = lo; i < hi; ++i) list[pos[key[i] / kchunk]++] = (int32_t)i;
     });
-    // each thread owns a range of keys and scans every particle in index
-    // order: the counts and placements of its keys are its own, and a cell
-    // keeps ascending indices
     start.assign((size_t)nkeys + 1, 0u);
-    host_parallel(nkeys, [&](int64_t klo, int64_t khi) {
-        for (int64_t i = 0; i < n; ++i) {
-            const int64_t k = key[i];
-            if (k >= klo && k < khi) ++start[k + 1];
-        }
-    });
-    for (int64_t k = 0; k < nkeys; ++k) start[k + 1] += start[k];
     order.resize(n);
-    host_parallel(nkeys, [&](int64_t klo, int64_t khi) {
-        for (int64_t i = 0; i < n; ++i) {
-            const int64_t k = key[i];
-            if (k >= klo && k < khi) order[start[k]++] = i;
+    std::vector<int64_t> own_total(16, 0);
+    auto each_own = [&](auto&& f) {
+        host_tasks(O, [&](int64_t o) {
+            for (int64_t r = 0; r < R; ++r) {
+                const int64_t* off = &ownoff[r * 17];
+                const std::vector<int32_t>& list = byown[r];
+                for (int64_t q = off[o]; q < off[o + 1]; ++q) f(list[q]);
+            }
+        });
+    };
+    // counts, then each owner's exclusive scan of its keys
+    each_own([&](int32_t i) { ++start[key[i]]; });
+    host_tasks(O, [&](int64_t o) {
+        const int64_t klo = o * kchunk, khi = std::min(nkeys, klo + kchunk);
+        int64_t acc = 0;
+        for (int64_t k = klo; k < khi; ++k) {
+            const int64_t c = start[k];
+            start[k] = (uint32_t)acc;
+            acc += c;
         }
+        own_total[o] = acc;
     });
+    std::vector<int64_t> base(17, 0);
+    for (int64_t o = 0; o < O; ++o) base[o + 1] = base[o] + own_total[o];
+    host_tasks(O, [&](int64_t o) {
+        const int64_t klo = o * kchunk, khi = std::min(nkeys, klo + kchunk);
+        for (int64_t k = klo; k < khi; ++k) start[k] += (uint32_t)base[o];
+    });
+    each_own([&](int32_t i) { order[start[key[i]]++] = i; });
     hr.type_idx.assign(type, type + n);
     hr.posh.resize(n); hr.vm.resize(n); hr.rho.resize(n); hr.e.resize(n); hr.c.resize(n);
     hr.p.resize(n); hr.meta.resize(n);
