@@ -3336,7 +3336,7 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     auto get4 = [&](double4* src, int which) -> int {  // which: 0 xyz, 1 w
         STAGE(double4, h, n);
         if (int rc_ = fetch(h, src, sizeof(double4))) return rc_;
-        for (int64_t s = 0; s < n; ++s) {
+        host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) {
             int64_t i = idx(s);
             if (which == 0) {
                 dst[i * dim] = h[s].x;
@@ -3345,13 +3345,13 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
             } else {
                 dst[i] = h[s].w;
             }
-        }
+        } });
         return 0;
     };
     auto get1 = [&](double* src) -> int {
         STAGE(double, h, n);
         if (int rc_ = fetch(h, src, sizeof(double))) return rc_;
-        for (int64_t s = 0; s < n; ++s) dst[idx(s)] = h[s];
+        host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) dst[idx(s)] = h[s]; });
         return 0;
     };
     auto getT = [&](double* src) -> int {
@@ -3362,14 +3362,13 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         }
         STAGE(double, h, n * nn);
         if (int rc_ = fetch(h, src, nn * sizeof(double))) return rc_;
-        for (int64_t s = 0; s < n; ++s)
-            for (int q = 0; q < nn; ++q) dst[idx(s) * nn + q] = h[s * nn + q];
+        host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) for (int q = 0; q < nn; ++q) dst[idx(s) * nn + q] = h[s * nn + q]; });
         return 0;
     };
     auto getf4 = [&](float4* src, int which) -> int {  // which: 0 xyz, 1 w
         STAGE(float4, h, n);
         if (int rc_ = fetch(h, src, sizeof(float4))) return rc_;
-        for (int64_t s = 0; s < n; ++s) {
+        host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) {
             int64_t i = idx(s);
             if (which == 0) {
                 dst[i * dim] = h[s].x;
@@ -3378,13 +3377,13 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
             } else {
                 dst[i] = h[s].w;
             }
-        }
+        } });
         return 0;
     };
     auto getf1 = [&](float* src, bool add_rho0) -> int {
         STAGE(float, h, n);
         if (int rc_ = fetch(h, src, sizeof(float))) return rc_;
-        for (int64_t s = 0; s < n; ++s) {
+        host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) {
             double r0 = 0.0;
             if (add_rho0) {
                 int a = std::abs(typ(s));
@@ -3392,7 +3391,7 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
                     if (cfg.mat[k].type == a) r0 = cfg.mat[k].rho0;
             }
             dst[idx(s)] = r0 + (double)h[s];
-        }
+        } });
         return 0;
     };
     if (mixed) {
@@ -3406,7 +3405,7 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
         case SPHGPU_F_DIVV: {
             STAGE(float2, h, n);
             if (int rc_ = fetch(h, D.divf, sizeof(float2))) return rc_;
-            for (int64_t s = 0; s < n; ++s) dst[idx(s)] = field == SPHGPU_F_DIVX ? h[s].x : h[s].y;
+            host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) dst[idx(s)] = field == SPHGPU_F_DIVX ? h[s].x : h[s].y; });
             return 0;
         }
         case SPHGPU_F_DEDT: return getf1(D.def, false);
@@ -3416,7 +3415,7 @@ static int download_view(const Dev& D, int64_t n, int dim, bool mixed, const sph
     auto get2 = [&](int which) -> int {
         STAGE(double2, h, n);
         if (int rc_ = fetch(h, D.div, sizeof(double2))) return rc_;
-        for (int64_t s = 0; s < n; ++s) dst[idx(s)] = which ? h[s].y : h[s].x;
+        host_parallel(n, [&](int64_t lo_, int64_t hi_) { for (int64_t s = lo_; s < hi_; ++s) dst[idx(s)] = which ? h[s].y : h[s].x; });
         return 0;
     };
     switch (field) {
