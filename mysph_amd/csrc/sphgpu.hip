@@ -1798,14 +1798,62 @@ struct Buf {
 // neighbours of a wavefront's lanes sit in consecutive slots and the gathers
 // coalesce. A context keeps its HostRun between uploads, so a host that
 // uploads every step (the drop-in single_step) reuses these buffers.
+// A host array in slot order for an upload. A context keeps its arrays
+// pinned (hipHostMalloc, grown on demand and reused by later uploads), so
+// their copies to the device are DMA from pinned memory; a HostRun that does
+// not ask for that, or that cannot get pinned memory, keeps them pageable.
+template <typename T>
+struct HostArr {
+    bool want_pinned = false;
+    T* pin = nullptr;
+    size_t pin_n = 0, n_ = 0;
+    std::vector<T> v;
+    HostArr() = default;
+    HostArr(const HostArr&) = delete;
+    HostArr& operator=(const HostArr&) = delete;
+    ~HostArr() { if (pin) hipHostFree(pin); }
+    void resize(size_t n) {
+        n_ = n;
+        if (want_pinned) {
+            if (n > pin_n) {
+                if (pin) hipHostFree(pin);
+                pin = nullptr;
+                pin_n = 0;
+                void* q = nullptr;
+                if (hipHostMalloc(&q, n * sizeof(T), hipHostMallocDefault) == hipSuccess) {
+                    pin = static_cast<T*>(q);
+                    pin_n = n;
+                } else {
+                    (void)hipGetLastError();   // pageable from here on
+                }
+            }
+            if (pin) return;
+        }
+        v.resize(n);
+    }
+    T* data() { return pin ? pin : v.data(); }
+    const T* data() const { return pin ? pin : v.data(); }
+    T& operator[](size_t i) { return data()[i]; }
+    const T& operator[](size_t i) const { return data()[i]; }
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    void clear() { n_ = 0; v.clear(); }
+    const T* begin() const { return data(); }
+    const T* end() const { return data() + n_; }
+};
+
 struct HostRun {
     Params P{};
     int64_t n = 0, n_real = 0, n_dummy = 0;
     bool fast = false;
-    std::vector<double4> posh, vm;
-    std::vector<double> rho, e, c, p, shear;   // shear empty when the host passes none
-    std::vector<int4> meta;
-    std::vector<float> drho;   // mixed precision: rho - rho0
+    HostArr<double4> posh, vm;
+    HostArr<double> rho, e, c, p, shear;   // shear empty when the host passes none
+    HostArr<int4> meta;
+    HostArr<float> drho;   // mixed precision: rho - rho0
+    void pin_arrays() {
+        posh.want_pinned = vm.want_pinned = rho.want_pinned = e.want_pinned = true;
+        c.want_pinned = p.want_pinned = shear.want_pinned = meta.want_pinned = drho.want_pinned = true;
+    }
     std::vector<uint32_t> sort_key, sort_start;   // counting-sort work space
     std::vector<int64_t> order;                   // slot -> particle index
     std::vector<int32_t> type_idx;                // type by particle index
@@ -2004,6 +2052,7 @@ extern "C" int sphgpu_create(const sphgpu_config* cfg, int64_t capacity, sphgpu_
     if (rc) return create_fail(nullptr, rc, why);
     sphgpu_ctx* c = new (std::nothrow) sphgpu_ctx();
     if (!c) return create_fail(nullptr, SPHGPU_E_ARG, "out of host memory");
+    c->up.pin_arrays();   // uploads stage in pinned memory kept by the context
     c->cfg = *cfg;
     c->dim = cfg->dim;
     c->capacity = capacity;
@@ -3121,14 +3170,14 @@ extern "C" int sphgpu_upload(sphgpu_ctx* c, int64_t n_real, int64_t n_dummy, con
     if (rc) return rc;
     // every state buffer starts at zero (alloc_state), so the displacement
     // and the derivatives need no copy
-    const std::vector<double4>& posh = hr.posh;
-    const std::vector<double4>& vm = hr.vm;
-    const std::vector<double>& vrho = hr.rho;
-    const std::vector<double>& ve = hr.e;
-    const std::vector<double>& vc = hr.c;
-    const std::vector<double>& vp = hr.p;
-    const std::vector<int4>& meta = hr.meta;
-    const std::vector<double>& vshear = hr.shear;
+    const auto& posh = hr.posh;
+    const auto& vm = hr.vm;
+    const auto& vrho = hr.rho;
+    const auto& ve = hr.e;
+    const auto& vc = hr.c;
+    const auto& vp = hr.p;
+    const auto& meta = hr.meta;
+    const auto& vshear = hr.shear;
     Dev& D = c->D;
     HIPCHK(c, hipMemcpyAsync(D.posh, posh.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(D.vm, vm.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
