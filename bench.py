@@ -332,8 +332,14 @@ def serving_leg(args):
 
     tuner = serving.Tuner()
     reps = {0: 10000, 1: 40000, 2: 150000, 3: 450000}
+    # candidates keep 3 cells per cutoff: the arena's cell room is sized for
+    # that, and a finer grid (4 per cutoff: 1.8x the cells in 2D) that times
+    # faster alone leaves room for fewer resident runs (17.4-17.7 runs/s at
+    # saturation against 18.4 with 3 everywhere). Timed over steps 501-700,
+    # where the flow has the speeds that set the rebuild rate of 2000-step runs.
     report = tuner.tune(make_batch, lambda k: serving.dam_break_request_case(reps[k], 1.25, 0.1),
-                        fill=2_000_000, warmup=5, steps=20)
+                        candidates=((3, 0.15), (3, 0.08), (3, 0.25)),
+                        fill=1_000_000, warmup=500, steps=200)
     log(f"serving: tuned tiling per size class {tuner.table}")
 
     def serve(rate):
